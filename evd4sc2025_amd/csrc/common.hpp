@@ -1,0 +1,213 @@
+// Shared host/device infrastructure of the eigensolver library: the context
+// (stream, zero-initialised arena, error text, timing events, tuning), status
+// helpers and small device utilities.  Everything is gfx950 / wave64 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/evd4sc.h"
+
+namespace evd {
+
+constexpr int kWave = 64;
+
+struct Tuning {
+  int bc_sweeps = 0;       // sweeps aggregated per BC-back block (0 = b)
+  int bc_waves = 0;        // pipelined waves per BC-back workgroup (0 = 4)
+  int chase_groups = 0;    // persistent workgroups of the bulge chase (0 = auto)
+  int sbr_back_block = 0;  // WY block width of the SBR back transformation (0 = 1024)
+};
+
+}  // namespace evd
+
+struct evd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cus = 0;
+
+  // arena: one device allocation, bump-allocated inside a top-level call
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+  size_t arena_top = 0;
+  size_t budget = 0;  // 0 = unlimited
+  int depth = 0;      // nesting depth of public entry points
+
+  // small always-resident control block: barrier words, error word, flags
+  unsigned *ctrl = nullptr;  // kCtrlWords words, zeroed at creation
+  size_t ctrl_words = 0;
+
+  evd::Tuning tune;
+  int precision = EVD_PREC_FP32;
+
+  bool timing = false;
+  hipEvent_t ev[EVD_STAGE_COUNT + 1] = {};
+  float stage_ms[EVD_STAGE_COUNT] = {};
+
+  char err[1024] = {0};
+};
+
+namespace evd {
+
+// control block layout (words)
+enum : int {
+  kCtrlError = 0,      // device-side error word (EVD_E_*), sticky until read
+  kCtrlBarrier = 16,   // grid barrier: [count, generation] pairs, 64 slots
+  kCtrlWordsTotal = 16 + 2 * 64,
+};
+
+int set_error(evd_ctx *ctx, int code, const char *fmt, ...);
+
+#define EVD_HIP(ctx, call)                                                                          \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess)                                                                           \
+      return ::evd::set_error((ctx), EVD_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                              __FILE__, __LINE__);                                                  \
+  } while (0)
+
+#define EVD_TRY(expr)        \
+  do {                       \
+    int s_ = (expr);         \
+    if (s_ != 0) return s_;  \
+  } while (0)
+
+#define EVD_ARG(ctx, cond, k)                                                                   \
+  do {                                                                                          \
+    if (!(cond)) return ::evd::set_error((ctx), -(k), "argument %d invalid: %s", (k), #cond); \
+  } while (0)
+
+// Launch check: kernel launch errors surface through hipGetLastError.
+#define EVD_LAUNCHED(ctx) EVD_HIP(ctx, hipGetLastError())
+
+// RAII frame over the arena: sub-allocations made while the frame lives are
+// released when it ends.
+struct ArenaFrame {
+  evd_ctx *ctx;
+  size_t mark;
+  explicit ArenaFrame(evd_ctx *c) : ctx(c), mark(c->arena_top) {}
+  ~ArenaFrame() { ctx->arena_top = mark; }
+};
+
+// Ensure the arena holds at least `bytes` beyond the current top.  Growing
+// reallocates, so it is only allowed when nothing is allocated (top == 0).
+int arena_ensure(evd_ctx *ctx, size_t bytes);
+// Bump allocation (256-byte aligned).  Returns nullptr and sets the error if
+// the arena is too small.
+void *arena_alloc(evd_ctx *ctx, size_t bytes);
+template <typename T>
+T *arena_alloc_n(evd_ctx *ctx, size_t n) {
+  return static_cast<T *>(arena_alloc(ctx, n * sizeof(T)));
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// fills (stream-ordered kernels; never hipMemsetAsync so calls stay
+// capturable in hipGraphs)
+int fill_f32(evd_ctx *ctx, float *p, long rows, long cols, long ld, float v);
+int fill_f64(evd_ctx *ctx, double *p, long rows, long cols, long ld, double v);
+int fill_u32(evd_ctx *ctx, unsigned *p, size_t n, unsigned v);
+int set_identity_f64(evd_ctx *ctx, double *p, long n, long ld);
+int copy_f32(evd_ctx *ctx, const float *src, long lds, float *dst, long ldd, long rows, long cols);
+
+// GEMM entry points (gemm.hip)
+int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, const float *A, long lda,
+          const float *B, long ldb, float beta, float *C, long ldc, long batch = 1, long sA = 0, long sB = 0,
+          long sC = 0);
+int dgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, double alpha, const double *A, long lda,
+          const double *B, long ldb, double beta, double *C, long ldc);
+int syr2k_lower_mirror(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
+                       float beta, float *C, long ldc);
+
+// grouped FP64 GEMM: C[:, colmap[j]] = A (m x k) * B (k x n) per problem
+struct DGemmProblem {
+  int m, n, k;
+  int tile_begin;  // prefix of tiles
+  const double *A;
+  long lda;
+  const double *B;
+  long ldb;
+  double *C;
+  long ldc;
+  const int *colmap;  // nullable: output column j goes to C column colmap[j]
+};
+int dgemm_grouped(evd_ctx *ctx, const DGemmProblem *dev_problems, int nproblems, int total_tiles);
+constexpr int kDGroupTile = 64;
+
+int max_coresident_blocks(evd_ctx *ctx, const void *kernel, int block, size_t dyn_smem, int *out);
+
+// error word readback (synchronises the stream)
+int check_device_error(evd_ctx *ctx, const char *where);
+
+}  // namespace evd
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+namespace evd {
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline unsigned ld_acquire(const unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_acquire(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_release(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 100 MHz constant clock -> bounded waits (watchdog) in persistent kernels
+__device__ inline unsigned long long clock100() { return wall_clock64(); }
+constexpr unsigned long long kWatchdogTicks = 100ull * 1000ull * 1000ull * 20ull;  // 20 s
+
+// Sense-free generation barrier over `nblocks` co-resident workgroups.
+// words[0] = arrival count (returns to 0), words[1] = generation.  Every
+// thread of every block must call it.  Returns false if the watchdog expired
+// (the caller must then leave the kernel).
+__device__ inline bool grid_barrier(unsigned *words, unsigned nblocks, unsigned *err) {
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_ok;
+  if (threadIdx.x == 0) {
+    s_ok = 1;
+    unsigned gen = __hip_atomic_load(&words[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned arrived = __hip_atomic_fetch_add(&words[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (arrived == nblocks) {
+      __hip_atomic_store(&words[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&words[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned long long t0 = clock100();
+      while (__hip_atomic_load(&words[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (clock100() - t0 > kWatchdogTicks) {
+          atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+          s_ok = 0;
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  __threadfence();
+  return s_ok != 0;
+}
+
+}  // namespace evd

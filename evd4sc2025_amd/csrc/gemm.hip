@@ -1,0 +1,486 @@
+// Dense MFMA kernels for gfx950.
+//
+//  * sgemm: C = alpha op(A) op(B) + beta C, FP32 in/out, v_mfma_f32_16x16x4f32
+//    with FP32 accumulation; tile shapes for square, tall-skinny (n <= 32) and
+//    short-wide (m <= 32) products; split-K with a deterministic two-pass
+//    reduction for the long-K / small-MN reductions of the ZY updates.
+//  * syr2k_lower_mirror: C = alpha (A B^T + B A^T) + beta C on lower tiles
+//    only, the result mirrored to the upper triangle in the epilogue
+//    (the role of tc_ozimmu_syr2k + CpyMatrixL2U in the reference).
+//  * dgemm / dgemm_grouped: FP64 on v_mfma_f64_16x16x4f64 for the
+//    divide-and-conquer merges and the FP64 metrics.
+//
+// Operand staging: global -> registers (prefetch of tile k+1) -> LDS (k-major,
+// the MFMA lane index running along the contiguous LDS dimension, 4-float
+// padding) -> MFMA.  Operands are passed to the MFMA as (B-fragment,
+// A-fragment) so that the accumulator lane index runs down a column of the
+// column-major C, which makes the epilogue stores contiguous.
+#include "common.hpp"
+
+namespace evd {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBK = 16;
+
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T>
+struct TileLoader {
+  static constexpr int NA = BM * kBK / 256;
+  static constexpr int NB = BN * kBK / 256;
+  static constexpr int LDSA = BM + 4;
+  static constexpr int LDSB = BN + 4;
+  T ra[NA], rb[NB];
+
+  __device__ inline void load(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M, int N,
+                              int m0, int n0, int k0, int ke, int tid) {
+#pragma unroll
+    for (int q = 0; q < NA; q++) {
+      int idx = tid + 256 * q;
+      int i, kk;
+      if (!TA) {
+        i = idx % BM;
+        kk = idx / BM;
+      } else {
+        kk = idx % kBK;
+        i = idx / kBK;
+      }
+      int gi = m0 + i, gk = k0 + kk;
+      T v = T(0);
+      if (gi < M && gk < ke) v = TA ? A[gk + (long)gi * lda] : A[gi + (long)gk * lda];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      int idx = tid + 256 * q;
+      int j, kk;
+      if (TB) {
+        j = idx % BN;
+        kk = idx / BN;
+      } else {
+        kk = idx % kBK;
+        j = idx / kBK;
+      }
+      int gj = n0 + j, gk = k0 + kk;
+      T v = T(0);
+      if (gj < N && gk < ke) v = TB ? B[gj + (long)gk * ldb] : B[gk + (long)gj * ldb];
+      rb[q] = v;
+    }
+  }
+
+  __device__ inline void store(T *As, T *Bs, int tid) {
+#pragma unroll
+    for (int q = 0; q < NA; q++) {
+      int idx = tid + 256 * q;
+      int i, kk;
+      if (!TA) {
+        i = idx % BM;
+        kk = idx / BM;
+      } else {
+        kk = idx % kBK;
+        i = idx / kBK;
+      }
+      As[kk * LDSA + i] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      int idx = tid + 256 * q;
+      int j, kk;
+      if (TB) {
+        j = idx % BN;
+        kk = idx / BN;
+      } else {
+        kk = idx % kBK;
+        j = idx / kBK;
+      }
+      Bs[kk * LDSB + j] = rb[q];
+    }
+  }
+};
+
+__device__ inline f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ inline f64x4 mfma16(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Core: accumulate op(A)[m0:m0+BM, kb:ke] * op(B)[kb:ke, n0:n0+BN] into acc.
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, typename VT>
+__device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M,
+                                     int N, int m0, int n0, int kb, int ke, T *As0, T *Bs0, VT (&acc)[BM / WM / 16][BN / WN / 16]) {
+  using L = TileLoader<TA, TB, BM, BN, WM, WN, T>;
+  constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
+  constexpr int SA = kBK * L::LDSA, SB = kBK * L::LDSB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  L ld;
+  int nk = ke > kb ? (ke - kb + kBK - 1) / kBK : 0;
+  if (nk > 0) {
+    ld.load(A, lda, B, ldb, M, N, m0, n0, kb, ke, tid);
+    ld.store(As0, Bs0, tid);
+  }
+  __syncthreads();
+  for (int t = 0; t < nk; t++) {
+    const int cur = t & 1;
+    if (t + 1 < nk) ld.load(A, lda, B, ldb, M, N, m0, n0, kb + (t + 1) * kBK, ke, tid);
+    const T *As = As0 + cur * SA;
+    const T *Bs = Bs0 + cur * SB;
+#pragma unroll
+    for (int ks = 0; ks < kBK / 4; ks++) {
+      T a[MI], b[NI];
+      const int kr = ks * 4 + (lane >> 4);
+#pragma unroll
+      for (int mi = 0; mi < MI; mi++) a[mi] = As[kr * L::LDSA + wm * (BM / WM) + mi * 16 + (lane & 15)];
+#pragma unroll
+      for (int ni = 0; ni < NI; ni++) b[ni] = Bs[kr * L::LDSB + wn * (BN / WN) + ni * 16 + (lane & 15)];
+#pragma unroll
+      for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+        for (int ni = 0; ni < NI; ni++) acc[mi][ni] = mfma16(b[ni], a[mi], acc[mi][ni]);
+    }
+    if (t + 1 < nk) ld.store(As0 + (cur ^ 1) * SA, Bs0 + (cur ^ 1) * SB, tid);
+    __syncthreads();
+  }
+}
+
+template <bool TA, bool TB, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float alpha, const float *__restrict__ A,
+                                                    long lda, long sA, const float *__restrict__ B, long ldb, long sB,
+                                                    float beta, float *__restrict__ C, long ldc, long sC, int kchunk,
+                                                    int ksplit, float *__restrict__ P, long batches) {
+  using L = TileLoader<TA, TB, BM, BN, WM, WN, float>;
+  constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
+  __shared__ float As[2 * kBK * L::LDSA];
+  __shared__ float Bs[2 * kBK * L::LDSB];
+  const long bz = blockIdx.z;
+  const long batch = bz / ksplit;
+  const int kz = (int)(bz % ksplit);
+  A += batch * sA;
+  B += batch * sB;
+  C += batch * sC;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kb = kz * kchunk;
+  const int ke = min(K, kb + kchunk);
+  f32x4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4>(A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++) {
+    const int row = m0 + wm * (BM / WM) + mi * 16 + (lane & 15);
+    if (row >= M) continue;
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        if (col >= N) continue;
+        float v = acc[mi][ni][r];
+        if (ksplit > 1) {
+          P[((long)kz * batches + batch) * (long)M * N + row + (long)col * M] = v;
+        } else {
+          long o = row + (long)col * ldc;
+          C[o] = (beta == 0.f) ? alpha * v : alpha * v + beta * C[o];
+        }
+      }
+    }
+  }
+}
+
+__global__ void splitk_reduce_kernel(int M, int N, long batches, int ksplit, const float *__restrict__ P, float alpha,
+                                     float beta, float *__restrict__ C, long ldc, long sC) {
+  long total = (long)M * N * batches;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    long b = t / ((long)M * N);
+    long rc = t % ((long)M * N);
+    int row = (int)(rc % M);
+    long col = rc / M;
+    float s = 0.f;
+    for (int kz = 0; kz < ksplit; kz++) s += P[((long)kz * batches + b) * (long)M * N + rc];
+    long o = b * sC + row + col * ldc;
+    C[o] = (beta == 0.f) ? alpha * s : alpha * s + beta * C[o];
+  }
+}
+
+template <bool TA, bool TB, int BM, int BN, int WM, int WN>
+static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
+                        const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch) {
+  long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
+  int ksplit = 1;
+  const long target = 2L * ctx->num_cus;
+  if (tiles < target && k >= 1024) {
+    long want = (target + tiles - 1) / tiles;
+    long maxs = k / 256;
+    ksplit = (int)std::min<long>(std::min<long>(want, maxs), 64);
+    if (ksplit < 1) ksplit = 1;
+  }
+  long kchunk = (k + ksplit - 1) / ksplit;
+  kchunk = (kchunk + kBK - 1) / kBK * kBK;
+  ksplit = (int)((k + kchunk - 1) / kchunk);
+  if (ksplit < 1) ksplit = 1;
+  float *P = nullptr;
+  ArenaFrame frame(ctx);
+  if (ksplit > 1) {
+    P = arena_alloc_n<float>(ctx, (size_t)ksplit * batch * m * n);
+    if (!P) return EVD_E_NOMEM;
+  }
+  dim3 grid((unsigned)((m + BM - 1) / BM), (unsigned)((n + BN - 1) / BN), (unsigned)(batch * ksplit));
+  sgemm_kernel<TA, TB, BM, BN, WM, WN><<<grid, 256, 0, ctx->stream>>>(
+      (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch);
+  EVD_LAUNCHED(ctx);
+  if (ksplit > 1) {
+    long total = m * n * batch;
+    unsigned g = (unsigned)std::min<long>((total + 255) / 256, 4096);
+    splitk_reduce_kernel<<<g, 256, 0, ctx->stream>>>((int)m, (int)n, batch, ksplit, P, alpha, beta, C, ldc, sC);
+    EVD_LAUNCHED(ctx);
+  }
+  return 0;
+}
+
+template <bool TA, bool TB>
+static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
+                          const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch) {
+  if (n <= 32 && m > 64)
+    return launch_sgemm<TA, TB, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  if (m <= 32 && n > 64)
+    return launch_sgemm<TA, TB, 32, 256, 1, 4>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  long big_tiles = ((m + 127) / 128) * ((n + 127) / 128) * batch;
+  if (big_tiles >= 2L * ctx->num_cus || (m > 64 && n > 64 && k < 1024))
+    return launch_sgemm<TA, TB, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  return launch_sgemm<TA, TB, 64, 64, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+__global__ void scale_kernel(int M, int N, float beta, float *C, long ldc, long sC, long batch) {
+  long total = (long)M * N * batch;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    long b = t / ((long)M * N);
+    long rc = t % ((long)M * N);
+    long o = b * sC + (rc % M) + (rc / M) * ldc;
+    C[o] = beta == 0.f ? 0.f : beta * C[o];
+  }
+}
+
+int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, const float *A, long lda,
+          const float *B, long ldb, float beta, float *C, long ldc, long batch, long sA, long sB, long sC) {
+  if (m <= 0 || n <= 0 || batch <= 0) return 0;
+  if (k <= 0 || alpha == 0.f) {
+    if (beta == 1.f) return 0;
+    long total = m * n * batch;
+    scale_kernel<<<(unsigned)std::min<long>((total + 255) / 256, 4096), 256, 0, ctx->stream>>>((int)m, (int)n, beta, C,
+                                                                                             ldc, sC, batch);
+    EVD_LAUNCHED(ctx);
+    return 0;
+  }
+  bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+  if (!TA && !TB) return sgemm_dispatch<false, false>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  if (TA && !TB) return sgemm_dispatch<true, false>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  if (!TA && TB) return sgemm_dispatch<false, true>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  return sgemm_dispatch<true, true>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+// ----------------------------------------------------------------- SYR2K
+// Lower tiles (ti >= tj) of C; the K loop runs over [A|B] against [B|A].
+__global__ __launch_bounds__(256) void syr2k_kernel(int n, int k, float alpha, const float *__restrict__ A, long lda,
+                                                    const float *__restrict__ B, long ldb, float beta,
+                                                    float *__restrict__ C, long ldc) {
+  constexpr int BM = 128, BN = 128, WM = 2, WN = 2, MI = 4, NI = 4;
+  using L = TileLoader<false, true, BM, BN, WM, WN, float>;
+  __shared__ float As[2 * kBK * L::LDSA];
+  __shared__ float Bs[2 * kBK * L::LDSB];
+  const long t = blockIdx.x;
+  long ti = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
+  while (ti * (ti + 1) / 2 > t) ti--;
+  const long tj = t - ti * (ti + 1) / 2;
+  const int m0 = (int)ti * BM, n0 = (int)tj * BN;
+  f32x4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // A_I * B_J^T  (op(B)(kk, j) = B[j + kk*ldb])
+  tile_mainloop<false, true, BM, BN, WM, WN, float, f32x4>(A, lda, B, ldb, n, n, m0, n0, 0, k, As, Bs, acc);
+  // B_I * A_J^T
+  tile_mainloop<false, true, BM, BN, WM, WN, float, f32x4>(B, ldb, A, lda, n, n, m0, n0, 0, k, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++) {
+    const int row = m0 + wm * (BM / WM) + mi * 16 + (lane & 15);
+    if (row >= n) continue;
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        if (col > row || col >= n) continue;
+        long o = row + (long)col * ldc;
+        float v = (beta == 0.f) ? alpha * acc[mi][ni][r] : alpha * acc[mi][ni][r] + beta * C[o];
+        C[o] = v;
+        if (col != row) C[col + (long)row * ldc] = v;
+      }
+  }
+}
+
+int syr2k_lower_mirror(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
+                       float beta, float *C, long ldc) {
+  if (n <= 0) return 0;
+  long nt = (n + 127) / 128;
+  long tiles = nt * (nt + 1) / 2;
+  syr2k_kernel<<<(unsigned)tiles, 256, 0, ctx->stream>>>((int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
+// ----------------------------------------------------------------- FP64
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void dgemm_kernel(int M, int N, int K, double alpha, const double *__restrict__ A,
+                                                    long lda, const double *__restrict__ B, long ldb, double beta,
+                                                    double *__restrict__ C, long ldc) {
+  constexpr int BM = 64, BN = 64, WM = 2, WN = 2, MI = 2, NI = 2;
+  using L = TileLoader<TA, TB, BM, BN, WM, WN, double>;
+  __shared__ double As[2 * kBK * L::LDSA];
+  __shared__ double Bs[2 * kBK * L::LDSB];
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  f64x4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f64x4{0., 0., 0., 0.};
+  tile_mainloop<TA, TB, BM, BN, WM, WN, double, f64x4>(A, lda, B, ldb, M, N, m0, n0, 0, K, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++) {
+    const int row = m0 + wm * (BM / WM) + mi * 16 + (lane & 15);
+    if (row >= M) continue;
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        if (col >= N) continue;
+        long o = row + (long)col * ldc;
+        C[o] = (beta == 0.0) ? alpha * acc[mi][ni][r] : alpha * acc[mi][ni][r] + beta * C[o];
+      }
+  }
+}
+
+int dgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, double alpha, const double *A, long lda,
+          const double *B, long ldb, double beta, double *C, long ldc) {
+  if (m <= 0 || n <= 0) return 0;
+  bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+  dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64));
+  if (k <= 0) {
+    // C = beta C through a zero-K product
+    k = 0;
+  }
+  if (!TA && !TB)
+    dgemm_kernel<false, false><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+  else if (TA && !TB)
+    dgemm_kernel<true, false><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+  else if (!TA && TB)
+    dgemm_kernel<false, true><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+  else
+    dgemm_kernel<true, true><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
+// Grouped FP64 GEMM: one workgroup per 64x64 output tile of one problem.
+__global__ __launch_bounds__(256) void dgemm_grouped_kernel(const DGemmProblem *__restrict__ probs, int nprob,
+                                                            int total_tiles) {
+  constexpr int BM = 64, BN = 64, WM = 2, WN = 2, MI = 2, NI = 2;
+  using L = TileLoader<false, false, BM, BN, WM, WN, double>;
+  __shared__ double As[2 * kBK * L::LDSA];
+  __shared__ double Bs[2 * kBK * L::LDSB];
+  const int t = blockIdx.x;
+  if (t >= total_tiles) return;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile_begin <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const DGemmProblem p = probs[lo];
+  const int local = t - p.tile_begin;
+  const int tm = (p.m + BM - 1) / BM;
+  if (p.m <= 0 || p.n <= 0) return;
+  const int m0 = (local % tm) * BM, n0 = (local / tm) * BN;
+  if (n0 >= p.n) return;
+  f64x4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f64x4{0., 0., 0., 0.};
+  tile_mainloop<false, false, BM, BN, WM, WN, double, f64x4>(p.A, p.lda, p.B, p.ldb, p.m, p.n, m0, n0, 0, p.k, As, Bs,
+                                                              acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++) {
+    const int row = m0 + wm * (BM / WM) + mi * 16 + (lane & 15);
+    if (row >= p.m) continue;
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        if (col >= p.n) continue;
+        const int oc = p.colmap ? p.colmap[col] : col;
+        p.C[row + (long)oc * p.ldc] = acc[mi][ni][r];
+      }
+  }
+}
+
+int dgemm_grouped(evd_ctx *ctx, const DGemmProblem *dev_problems, int nproblems, int total_tiles) {
+  if (nproblems <= 0 || total_tiles <= 0) return 0;
+  dgemm_grouped_kernel<<<(unsigned)total_tiles, 256, 0, ctx->stream>>>(dev_problems, nproblems, total_tiles);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
+}  // namespace evd
+
+using namespace evd;
+
+extern "C" int evd_sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, const float *A, long lda,
+                         const float *B, long ldb, float beta, float *C, long ldc) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, ta == 'N' || ta == 'T' || ta == 'n' || ta == 't', 2);
+  EVD_ARG(ctx, tb == 'N' || tb == 'T' || tb == 'n' || tb == 't', 3);
+  EVD_ARG(ctx, m >= 0, 4);
+  EVD_ARG(ctx, n >= 0, 5);
+  EVD_ARG(ctx, k >= 0, 6);
+  EVD_ARG(ctx, lda >= 1, 9);
+  EVD_ARG(ctx, ldb >= 1, 11);
+  EVD_ARG(ctx, ldc >= std::max<long>(1, m), 14);
+  // split-K partials live in the arena
+  size_t need = 0;
+  if (ctx->arena_top == 0) {
+    need = (size_t)64 * m * n * sizeof(float);
+    if (need > ((size_t)1 << 30)) need = (size_t)1 << 30;
+    EVD_TRY(arena_ensure(ctx, need));
+  }
+  return sgemm(ctx, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int evd_dgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, double alpha, const double *A,
+                         long lda, const double *B, long ldb, double beta, double *C, long ldc) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, ta == 'N' || ta == 'T' || ta == 'n' || ta == 't', 2);
+  EVD_ARG(ctx, tb == 'N' || tb == 'T' || tb == 'n' || tb == 't', 3);
+  EVD_ARG(ctx, m >= 0, 4);
+  EVD_ARG(ctx, n >= 0, 5);
+  EVD_ARG(ctx, k >= 0, 6);
+  EVD_ARG(ctx, ldc >= std::max<long>(1, m), 14);
+  return dgemm(ctx, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}

@@ -1,0 +1,665 @@
+// Symmetric tridiagonal divide and conquer on the GPU, FP64 throughout.
+// Replaces the host LAPACKE_dstedc('I') call of the reference driver
+// (src/EVD/myEVD.cu:529).  The mathematics is Cuppen's method as published
+// (Cuppen 1981; Gu & Eisenstat 1995) in the form LAPACK's dlaed0/1/2/3/4 use:
+//
+//  * tearing: T = diag(T1 - |e| e_k e_k^T, T2 - |e| e_1 e_1^T) + |e| v v^T,
+//    v = [e_k; sign(e) e_1], applied at every split up front;
+//  * merge: d = [lambda(T1); lambda(T2)], z = [last row of Q1; sign(e) first
+//    row of Q2] / sqrt(2), rho = 2|e|; sort d; deflate small |rho z_j| and
+//    close pairs (Givens rotation of the two columns, tol = 8 eps max(|d|,|z|));
+//  * secular equation 1 + rho sum z_j^2/(d_j - lambda) = 0 per root, solved
+//    relative to the nearer pole with a two-pole rational model step inside a
+//    bisection bracket;
+//  * Loewner recomputation of z (Gu-Eisenstat) for orthogonal vectors;
+//  * Q_new = Q' [U 0; 0 I] by grouped FP64 MFMA GEMMs over the top and bottom
+//    row blocks (column types 1/2/3 as in dlaed2/dlaed3).
+//
+// GPU layout: the binary split tree is built on the host; all merges of one
+// tree height run in the same launches (grid.y = merge), every merge working
+// in the index range [start, start+N) of n-long scratch arrays and in the
+// diagonal block of the n x n Q / scratch matrices.  The deflation scan is
+// sequential per merge (one thread), everything else is parallel.
+#include <algorithm>
+#include <cfloat>
+
+#include "common.hpp"
+
+namespace evd {
+
+struct MergeDesc {
+  int start, n1, n2, pad;
+  double rho;  // |e| at the split
+  double sgn;  // sign(e)
+};
+struct MergeOut {
+  int K, K1, K2, K3, ndefl, nrot;
+  double rho2;  // 2|e|
+  double zsum;  // sum of squares of non-deflated z
+};
+
+struct DCScratch {
+  double *ds, *zs;     // sorted d, z
+  int *cs;             // sorted -> local column
+  double *nd_d, *nd_z; // non-deflated
+  int *nd_col, *qpos;
+  double *df_d;
+  int *df_col;
+  int *rot_p, *rot_n;
+  double *rot_c, *rot_s;
+  int *types;
+  int *orig;
+  double *tau, *lam, *zhat;
+  int *colmap, *dmap;
+  double *Qc, *Uc;  // n x n, ld n
+};
+
+__global__ void dc_tear_kernel(const MergeDesc *md, int nm, double *D) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nm) return;
+  MergeDesc m = md[t];
+  int k = m.start + m.n1;
+  D[k - 1] -= m.rho;
+  D[k] -= m.rho;
+}
+
+__global__ void dc_sort_kernel(const MergeDesc *__restrict__ md, const double *__restrict__ D,
+                               const double *__restrict__ Z, long ldz, DCScratch sc) {
+  const MergeDesc m = md[blockIdx.y];
+  const int N = m.n1 + m.n2, st = m.start;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+    double d = D[st + t], z;
+    int pos;
+    if (t < m.n1) {
+      z = Z[(st + m.n1 - 1) + (long)(st + t) * ldz];
+      int lo = 0, hi = m.n2;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (D[st + m.n1 + mid] < d)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      pos = t + lo;
+    } else {
+      z = m.sgn * Z[(st + m.n1) + (long)(st + t) * ldz];
+      int lo = 0, hi = m.n1;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (D[st + mid] <= d)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      pos = (t - m.n1) + lo;
+    }
+    sc.ds[st + pos] = d;
+    sc.zs[st + pos] = z * 0.70710678118654752440;
+    sc.cs[st + pos] = t;
+  }
+}
+
+// One thread per merge: deflation scan in sorted order (dlaed2 logic).
+__global__ void dc_deflate_kernel(const MergeDesc *__restrict__ md, MergeOut *__restrict__ mo, DCScratch sc) {
+  if (threadIdx.x != 0) return;
+  const MergeDesc m = md[blockIdx.x];
+  const int N = m.n1 + m.n2, st = m.start;
+  const double rho = 2.0 * m.rho;
+  double *ds = sc.ds + st, *zs = sc.zs + st;
+  const int *cs = sc.cs + st;
+  int *types = sc.types + st;
+  double *nd_d = sc.nd_d + st, *nd_z = sc.nd_z + st;
+  int *nd_col = sc.nd_col + st, *qpos = sc.qpos + st;
+  double *df_d = sc.df_d + st;
+  int *df_col = sc.df_col + st;
+  int *rot_p = sc.rot_p + st, *rot_n = sc.rot_n + st;
+  double *rot_c = sc.rot_c + st, *rot_s = sc.rot_s + st;
+  for (int i = 0; i < N; i++) types[i] = (i < m.n1) ? 1 : 3;
+  double dmax = 0.0, zmax = 0.0;
+  for (int i = 0; i < N; i++) {
+    dmax = fmax(dmax, fabs(ds[i]));
+    zmax = fmax(zmax, fabs(zs[i]));
+  }
+  const double eps = DBL_EPSILON * 0.5;
+  const double tol = 8.0 * eps * fmax(dmax, zmax);
+  int K = 0, nd = 0, nrot = 0;
+  if (rho * zmax <= tol) {
+    for (int j = 0; j < N; j++) {
+      df_d[nd] = ds[j];
+      df_col[nd] = cs[j];
+      nd++;
+    }
+  } else {
+    int pj = -1;
+    for (int j = 0; j < N; j++) {
+      if (rho * fabs(zs[j]) <= tol) {
+        df_d[nd] = ds[j];
+        df_col[nd] = cs[j];
+        nd++;
+        continue;
+      }
+      if (pj < 0) {
+        pj = j;
+        continue;
+      }
+      double s = zs[pj], c = zs[j];
+      const double tau = hypot(c, s);
+      const double t = ds[j] - ds[pj];
+      c /= tau;
+      s = -s / tau;
+      if (fabs(t * c * s) <= tol) {
+        zs[j] = tau;
+        zs[pj] = 0.0;
+        const int cp = cs[pj], cn = cs[j];
+        if (types[cp] != types[cn]) types[cn] = 2;
+        types[cp] = 4;
+        rot_p[nrot] = cp;
+        rot_n[nrot] = cn;
+        rot_c[nrot] = c;
+        rot_s[nrot] = s;
+        nrot++;
+        const double tt = ds[pj] * c * c + ds[j] * s * s;
+        ds[j] = ds[pj] * s * s + ds[j] * c * c;
+        ds[pj] = tt;
+        df_d[nd] = tt;
+        df_col[nd] = cp;
+        nd++;
+        pj = j;
+      } else {
+        nd_d[K] = ds[pj];
+        nd_z[K] = zs[pj];
+        nd_col[K] = cs[pj];
+        K++;
+        pj = j;
+      }
+    }
+    if (pj >= 0) {
+      nd_d[K] = ds[pj];
+      nd_z[K] = zs[pj];
+      nd_col[K] = cs[pj];
+      K++;
+    }
+  }
+  // deflated list ascending (insertion sort; nearly sorted already)
+  for (int i = 1; i < nd; i++) {
+    double v = df_d[i];
+    int c = df_col[i];
+    int j = i - 1;
+    while (j >= 0 && df_d[j] > v) {
+      df_d[j + 1] = df_d[j];
+      df_col[j + 1] = df_col[j];
+      j--;
+    }
+    df_d[j + 1] = v;
+    df_col[j + 1] = c;
+  }
+  int K1 = 0, K2 = 0, K3 = 0;
+  double zsum = 0.0;
+  for (int k = 0; k < K; k++) {
+    int ty = types[nd_col[k]];
+    K1 += (ty == 1);
+    K2 += (ty == 2);
+    K3 += (ty == 3);
+    zsum += nd_z[k] * nd_z[k];
+  }
+  int p1 = 0, p2 = K1, p3 = K1 + K2;
+  for (int k = 0; k < K; k++) {
+    int ty = types[nd_col[k]];
+    qpos[k] = (ty == 1) ? p1++ : (ty == 2) ? p2++ : p3++;
+  }
+  MergeOut o;
+  o.K = K;
+  o.K1 = K1;
+  o.K2 = K2;
+  o.K3 = K3;
+  o.ndefl = nd;
+  o.nrot = nrot;
+  o.rho2 = rho;
+  o.zsum = zsum;
+  mo[blockIdx.x] = o;
+}
+
+// Row-parallel: apply the deflation rotations to Q's columns and gather the
+// permuted copy Qc = [non-deflated by type | deflated].
+__global__ void dc_rotate_gather_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo,
+                                        double *__restrict__ Z, long ldz, DCScratch sc, long ldq) {
+  const MergeDesc m = md[blockIdx.y];
+  const MergeOut o = mo[blockIdx.y];
+  const int N = m.n1 + m.n2, st = m.start;
+  double *Zb = Z + st + (long)st * ldz;
+  double *Qc = sc.Qc + st + (long)st * ldq;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+    for (int t = 0; t < o.nrot; t++) {
+      const int p = sc.rot_p[st + t], q = sc.rot_n[st + t];
+      const double c = sc.rot_c[st + t], s = sc.rot_s[st + t];
+      const double x = Zb[r + (long)p * ldz], y = Zb[r + (long)q * ldz];
+      Zb[r + (long)p * ldz] = c * x + s * y;
+      Zb[r + (long)q * ldz] = c * y - s * x;
+    }
+    for (int k = 0; k < o.K; k++) Qc[r + (long)sc.qpos[st + k] * ldq] = Zb[r + (long)sc.nd_col[st + k] * ldz];
+    for (int t = 0; t < o.ndefl; t++) Qc[r + (long)(o.K + t) * ldq] = Zb[r + (long)sc.df_col[st + t] * ldz];
+  }
+}
+
+// f(tau) pieces relative to pole `org`: psi over poles <= lower, phi over the rest.
+__device__ inline void secular_eval(const double *__restrict__ d, const double *__restrict__ z, int K, int org,
+                                    int lower, double tau, double rho, double &f, double &psi, double &dpsi,
+                                    double &phi, double &dphi) {
+  const double dorg = d[org];
+  psi = dpsi = phi = dphi = 0.0;
+  for (int k = 0; k < K; k++) {
+    const double del = (d[k] - dorg) - tau;
+    const double zk = z[k];
+    const double t = zk / del;
+    const double term = zk * t;
+    const double dterm = t * t;
+    if (k <= lower) {
+      psi += term;
+      dpsi += dterm;
+    } else {
+      phi += term;
+      dphi += dterm;
+    }
+  }
+  psi *= rho;
+  dpsi *= rho;
+  phi *= rho;
+  dphi *= rho;
+  f = 1.0 + psi + phi;
+}
+
+__global__ void dc_secular_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo, DCScratch sc,
+                                  unsigned *err) {
+  const MergeDesc m = md[blockIdx.y];
+  const MergeOut o = mo[blockIdx.y];
+  const int st = m.start, K = o.K;
+  const double *d = sc.nd_d + st, *z = sc.nd_z + st;
+  const double rho = o.rho2;
+  const double eps = DBL_EPSILON * 0.5;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K; i += gridDim.x * blockDim.x) {
+    int org;
+    double lo, hi;
+    if (i < K - 1) {
+      const double gap = d[i + 1] - d[i];
+      const double mid = 0.5 * gap;
+      double f, psi, dpsi, phi, dphi;
+      secular_eval(d, z, K, i, i, mid, rho, f, psi, dpsi, phi, dphi);
+      if (f >= 0.0) {
+        org = i;
+        lo = 0.0;
+        hi = mid;
+      } else {
+        org = i + 1;
+        lo = mid - gap;  // relative to d[i+1]
+        hi = 0.0;
+      }
+    } else {
+      org = K - 1;
+      lo = 0.0;
+      hi = rho * o.zsum;
+      if (hi <= 0.0) hi = rho;
+    }
+    const int lower = i;  // poles d[0..i] lie below the root
+    double tau = 0.5 * (lo + hi);
+    bool conv = false;
+    for (int it = 0; it < 200; it++) {
+      double f, psi, dpsi, phi, dphi;
+      secular_eval(d, z, K, org, lower, tau, rho, f, psi, dpsi, phi, dphi);
+      if (f < 0.0)
+        lo = tau;
+      else
+        hi = tau;
+      const double erretm = 10.0 * (fabs(psi) + fabs(phi)) + 4.0;
+      if (fabs(f) <= eps * erretm || (hi - lo) <= 4.0 * eps * fmax(fabs(lo), fabs(hi))) {
+        conv = true;
+        break;
+      }
+      // two-pole rational model around the current point
+      const double dl = (d[i] - d[org]) - tau;  // < 0
+      double tn;
+      bool ok = false;
+      if (i < K - 1) {
+        const double du = (d[i + 1] - d[org]) - tau;  // > 0
+        const double q = dl * dl * dpsi, s = du * du * dphi;
+        const double C = f - dl * dpsi - du * dphi;
+        const double a2 = C, a1 = -(C * (dl + du) + q + s), a0 = C * dl * du + q * du + s * dl;
+        double eta = 0.0;
+        if (fabs(a2) < 1e-300) {
+          if (a1 != 0.0) {
+            eta = -a0 / a1;
+            ok = true;
+          }
+        } else {
+          const double disc = a1 * a1 - 4.0 * a2 * a0;
+          if (disc >= 0.0) {
+            const double sq = sqrt(disc);
+            const double qq = -0.5 * (a1 + (a1 >= 0.0 ? sq : -sq));
+            const double r1 = qq / a2;
+            const double r2 = (qq != 0.0) ? a0 / qq : r1;
+            const bool in1 = (r1 > dl && r1 < du), in2 = (r2 > dl && r2 < du);
+            if (in1 && (!in2 || fabs(r1) <= fabs(r2))) {
+              eta = r1;
+              ok = true;
+            } else if (in2) {
+              eta = r2;
+              ok = true;
+            }
+          }
+        }
+        tn = tau + eta;
+      } else {
+        // single pole below: C + q/(dl - eta) = 0
+        const double q = dl * dl * dpsi;
+        const double C = f - dl * dpsi;
+        if (C > 0.0) {
+          tn = tau + dl + q / C;
+          ok = true;
+        } else {
+          tn = tau;
+        }
+      }
+      if (!ok || !(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+      if (tn == tau || tn <= lo || tn >= hi) {
+        // the bracket cannot be split further in FP64: tau is as good as it gets
+        conv = true;
+        break;
+      }
+      tau = tn;
+    }
+    if (!conv) atomicOr(err, (unsigned)EVD_E_NOCONVERGE);
+    sc.orig[st + i] = org;
+    sc.tau[st + i] = tau;
+    sc.lam[st + i] = d[org] + tau;
+  }
+}
+
+// zhat_j^2 = prod_i (lambda_i - d_j) / (rho prod_{k != j} (d_k - d_j))
+__global__ void dc_loewner_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo, DCScratch sc) {
+  const MergeDesc m = md[blockIdx.y];
+  const MergeOut o = mo[blockIdx.y];
+  const int st = m.start, K = o.K;
+  const double *d = sc.nd_d + st, *z = sc.nd_z + st;
+  const int *orig = sc.orig + st;
+  const double *tau = sc.tau + st;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < K; j += gridDim.x * blockDim.x) {
+    const double dj = d[j];
+    double prod = ((d[orig[j]] - dj) + tau[j]) / o.rho2;  // (lambda_j - d_j)/rho
+    for (int k = 0; k < K; k++) {
+      if (k == j) continue;
+      const double lk = (d[orig[k]] - dj) + tau[k];  // lambda_k - d_j
+      prod *= lk / (d[k] - dj);
+    }
+    const double zh = sqrt(fabs(prod));
+    sc.zhat[st + j] = (z[j] >= 0.0) ? zh : -zh;
+  }
+}
+
+// One workgroup per eigenvector column i of one merge: Uc[qpos[j], i].
+__global__ __launch_bounds__(256) void dc_vectors_kernel(const MergeDesc *__restrict__ md,
+                                                         const MergeOut *__restrict__ mo, DCScratch sc, long ldu) {
+  const MergeDesc m = md[blockIdx.y];
+  const MergeOut o = mo[blockIdx.y];
+  const int st = m.start, K = o.K;
+  const int i = blockIdx.x;
+  if (i >= K) return;
+  const double *d = sc.nd_d + st;
+  const double dorg = d[sc.orig[st + i]];
+  const double ti = sc.tau[st + i];
+  double *Uc = sc.Uc + st + (long)st * ldu;
+  __shared__ double red[4];
+  double ss = 0.0;
+  for (int j = threadIdx.x; j < K; j += 256) {
+    const double v = sc.zhat[st + j] / ((d[j] - dorg) - ti);
+    ss += v * v;
+  }
+  ss = wave_sum(ss);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  const double inv = 1.0 / sqrt(red[0] + red[1] + red[2] + red[3]);
+  for (int j = threadIdx.x; j < K; j += 256) {
+    const double v = sc.zhat[st + j] / ((d[j] - dorg) - ti);
+    Uc[sc.qpos[st + j] + (long)i * ldu] = v * inv;
+  }
+}
+
+// Final ascending order of [lambda (K) ; deflated (ndefl)], new D and column maps.
+__global__ void dc_order_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo, DCScratch sc,
+                                double *__restrict__ D) {
+  const MergeDesc m = md[blockIdx.y];
+  const MergeOut o = mo[blockIdx.y];
+  const int st = m.start, K = o.K, nd = o.ndefl;
+  const double *lam = sc.lam + st, *df = sc.df_d + st;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < K + nd; t += gridDim.x * blockDim.x) {
+    if (t < K) {
+      const double v = lam[t];
+      int lo = 0, hi = nd;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (df[mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const int pos = t + lo;
+      D[st + pos] = v;
+      sc.colmap[st + t] = st + pos;
+    } else {
+      const int u = t - K;
+      const double v = df[u];
+      int lo = 0, hi = K;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (lam[mid] <= v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const int pos = u + lo;
+      D[st + pos] = v;
+      sc.dmap[st + u] = st + pos;
+    }
+  }
+}
+
+__global__ void dc_copy_deflated_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo,
+                                        DCScratch sc, long ldq, double *__restrict__ Z, long ldz) {
+  const MergeDesc m = md[blockIdx.y];
+  const MergeOut o = mo[blockIdx.y];
+  const int st = m.start, N = m.n1 + m.n2;
+  const long total = (long)N * o.ndefl;
+  const double *Qc = sc.Qc + st + (long)st * ldq;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % N);
+    const int u = (int)(e / N);
+    Z[(st + r) + (long)sc.dmap[st + u] * ldz] = Qc[r + (long)(o.K + u) * ldq];
+  }
+}
+
+struct TreeNode {
+  int start, size, left, right, height;
+};
+
+static int build_tree(std::vector<TreeNode> &nodes, int start, int size) {
+  int id = (int)nodes.size();
+  nodes.push_back({start, size, -1, -1, 0});
+  if (size > 1) {
+    int n1 = size / 2;
+    int l = build_tree(nodes, start, n1);
+    int r = build_tree(nodes, start + n1, size - n1);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    nodes[id].height = 1 + std::max(nodes[l].height, nodes[r].height);
+  }
+  return id;
+}
+
+size_t stedc_workspace(long n) {
+  size_t b = 0;
+  b += align256((size_t)n * n * sizeof(double)) * 2;  // Qc, Uc
+  b += align256((size_t)n * sizeof(double)) * 12;
+  b += align256((size_t)n * sizeof(int)) * 12;
+  b += align256((size_t)n * (sizeof(MergeDesc) + sizeof(MergeOut)));
+  b += align256((size_t)2 * n * sizeof(DGemmProblem));
+  return b + (1 << 16);
+}
+
+int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
+  if (n <= 0) return 0;
+  if (n == 1) {
+    EVD_TRY(set_identity_f64(ctx, Z, 1, ldz));
+    return 0;
+  }
+  ArenaFrame frame(ctx);
+  DCScratch sc;
+  const long ldq = n;
+  sc.Qc = arena_alloc_n<double>(ctx, (size_t)n * n);
+  sc.Uc = arena_alloc_n<double>(ctx, (size_t)n * n);
+  double **dbufs[] = {&sc.ds, &sc.zs, &sc.nd_d, &sc.nd_z, &sc.df_d, &sc.rot_c, &sc.rot_s, &sc.tau, &sc.lam, &sc.zhat};
+  for (auto p : dbufs) *p = arena_alloc_n<double>(ctx, (size_t)n);
+  int **ibufs[] = {&sc.cs, &sc.nd_col, &sc.qpos, &sc.df_col, &sc.rot_p, &sc.rot_n, &sc.types, &sc.orig, &sc.colmap, &sc.dmap};
+  for (auto p : ibufs) *p = arena_alloc_n<int>(ctx, (size_t)n);
+  if (!sc.Qc || !sc.Uc || !sc.zhat || !sc.dmap) return EVD_E_NOMEM;
+
+  // ---- host tree, merges grouped by height
+  std::vector<TreeNode> nodes;
+  nodes.reserve(2 * n);
+  build_tree(nodes, 0, (int)n);
+  const int H = nodes[0].height;
+  std::vector<std::vector<int>> by_height(H + 1);
+  for (int i = 0; i < (int)nodes.size(); i++)
+    if (nodes[i].size > 1) by_height[nodes[i].height].push_back(i);
+  std::vector<double> hE(n - 1);
+  EVD_HIP(ctx, hipMemcpyAsync(hE.data(), E, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, ctx->stream));
+  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<MergeDesc> all;
+  std::vector<int> level_off(H + 2, 0);
+  for (int h = 1; h <= H; h++) {
+    level_off[h] = (int)all.size();
+    for (int id : by_height[h]) {
+      const TreeNode &t = nodes[id];
+      MergeDesc m;
+      m.start = t.start;
+      m.n1 = nodes[t.left].size;
+      m.n2 = nodes[t.right].size;
+      m.pad = 0;
+      double e = hE[t.start + m.n1 - 1];
+      m.rho = fabs(e);
+      m.sgn = (e >= 0.0) ? 1.0 : -1.0;
+      all.push_back(m);
+    }
+  }
+  level_off[H + 1] = (int)all.size();
+  const int nmerge = (int)all.size();
+  MergeDesc *dmd = static_cast<MergeDesc *>(arena_alloc(ctx, sizeof(MergeDesc) * nmerge));
+  MergeOut *dmo = static_cast<MergeOut *>(arena_alloc(ctx, sizeof(MergeOut) * nmerge));
+  DGemmProblem *dprob = static_cast<DGemmProblem *>(arena_alloc(ctx, sizeof(DGemmProblem) * 2 * nmerge));
+  if (!dmd || !dmo || !dprob) return EVD_E_NOMEM;
+  EVD_HIP(ctx, hipMemcpyAsync(dmd, all.data(), sizeof(MergeDesc) * nmerge, hipMemcpyHostToDevice, ctx->stream));
+  unsigned *err = ctx->ctrl + kCtrlError;
+
+  // tearing of every split, then Q = I
+  dc_tear_kernel<<<(nmerge + 255) / 256, 256, 0, ctx->stream>>>(dmd, nmerge, D);
+  EVD_LAUNCHED(ctx);
+  EVD_TRY(set_identity_f64(ctx, Z, n, ldz));
+
+  std::vector<MergeOut> hmo(nmerge);
+  std::vector<DGemmProblem> probs;
+  for (int h = 1; h <= H; h++) {
+    const int off = level_off[h], nm = level_off[h + 1] - off;
+    if (nm == 0) continue;
+    const MergeDesc *lmd = dmd + off;
+    MergeOut *lmo = dmo + off;
+    int maxN = 0;
+    for (int i = 0; i < nm; i++) maxN = std::max(maxN, all[off + i].n1 + all[off + i].n2);
+    const unsigned gx = (unsigned)std::min(64, (maxN + 255) / 256);
+    dc_sort_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, D, Z, ldz, sc);
+    EVD_LAUNCHED(ctx);
+    dc_deflate_kernel<<<nm, 64, 0, ctx->stream>>>(lmd, lmo, sc);
+    EVD_LAUNCHED(ctx);
+    EVD_HIP(ctx, hipMemcpyAsync(hmo.data() + off, lmo, sizeof(MergeOut) * nm, hipMemcpyDeviceToHost, ctx->stream));
+    EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int maxK = 0;
+    long maxDefl = 0;
+    for (int i = 0; i < nm; i++) {
+      maxK = std::max(maxK, hmo[off + i].K);
+      maxDefl = std::max<long>(maxDefl, (long)hmo[off + i].ndefl * (all[off + i].n1 + all[off + i].n2));
+    }
+    dc_rotate_gather_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, lmo, Z, ldz, sc, ldq);
+    EVD_LAUNCHED(ctx);
+    if (maxK > 0) {
+      const unsigned gk = (unsigned)std::min(256, (maxK + 255) / 256);
+      dc_secular_kernel<<<dim3(gk, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, err);
+      EVD_LAUNCHED(ctx);
+      dc_loewner_kernel<<<dim3(gk, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc);
+      EVD_LAUNCHED(ctx);
+      dc_vectors_kernel<<<dim3(maxK, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq);
+      EVD_LAUNCHED(ctx);
+    }
+    dc_order_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, D);
+    EVD_LAUNCHED(ctx);
+    // grouped GEMMs: top rows use Qc[:, 0:K1+K2], bottom rows Qc[:, K1:K]
+    probs.clear();
+    int tiles = 0;
+    for (int i = 0; i < nm; i++) {
+      const MergeDesc &m = all[off + i];
+      const MergeOut &o = hmo[off + i];
+      if (o.K == 0) continue;
+      const int st = m.start;
+      DGemmProblem top;
+      top.m = m.n1;
+      top.n = o.K;
+      top.k = o.K1 + o.K2;
+      top.A = sc.Qc + st + (long)st * ldq;
+      top.lda = ldq;
+      top.B = sc.Uc + st + (long)st * ldq;
+      top.ldb = ldq;
+      top.C = Z + st;
+      top.ldc = ldz;
+      top.colmap = sc.colmap + st;
+      top.tile_begin = tiles;
+      tiles += ((top.m + kDGroupTile - 1) / kDGroupTile) * ((top.n + kDGroupTile - 1) / kDGroupTile);
+      probs.push_back(top);
+      DGemmProblem bot;
+      bot.m = m.n2;
+      bot.n = o.K;
+      bot.k = o.K2 + o.K3;
+      bot.A = sc.Qc + (st + m.n1) + (long)(st + o.K1) * ldq;
+      bot.lda = ldq;
+      bot.B = sc.Uc + (st + o.K1) + (long)st * ldq;
+      bot.ldb = ldq;
+      bot.C = Z + st + m.n1;
+      bot.ldc = ldz;
+      bot.colmap = sc.colmap + st;
+      bot.tile_begin = tiles;
+      tiles += ((bot.m + kDGroupTile - 1) / kDGroupTile) * ((bot.n + kDGroupTile - 1) / kDGroupTile);
+      probs.push_back(bot);
+    }
+    if (!probs.empty()) {
+      EVD_HIP(ctx, hipMemcpyAsync(dprob, probs.data(), sizeof(DGemmProblem) * probs.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+      EVD_TRY(dgemm_grouped(ctx, dprob, (int)probs.size(), tiles));
+    }
+    if (maxDefl > 0) {
+      const unsigned gd = (unsigned)std::min<long>((maxDefl + 255) / 256, 1024);
+      dc_copy_deflated_kernel<<<dim3(gd, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq, Z, ldz);
+      EVD_LAUNCHED(ctx);
+    }
+  }
+  return 0;
+}
+
+}  // namespace evd
+
+using namespace evd;
+
+extern "C" int evd_stedc_f64(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, n >= 0, 2);
+  EVD_ARG(ctx, D != nullptr || n == 0, 3);
+  EVD_ARG(ctx, E != nullptr || n <= 1, 4);
+  EVD_ARG(ctx, Z != nullptr || n == 0, 5);
+  EVD_ARG(ctx, ldz >= std::max<long>(1, n), 6);
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, stedc_workspace(n)));
+  EVD_TRY(stedc(ctx, n, D, E, Z, ldz));
+  return check_device_error(ctx, "evd_stedc_f64");
+}

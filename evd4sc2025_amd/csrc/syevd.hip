@@ -1,0 +1,300 @@
+// Whole pipeline SBR -> bulge chasing -> tridiagonal solve -> back
+// transformation (the stage sequence of main in src/EVD/myEVD.cu:251-571),
+// plus the accuracy metrics of util/tool/checkMetric.cu:5-25 and
+// src/EVD/myEVD.cu:598-639 computed in FP64 on the GPU.
+//
+// Differences from the reference driver, by design:
+//  * the tridiagonal stage runs on the GPU (FP64 divide and conquer) instead
+//    of host MKL; eigenvalues-only requests use Sturm-count bisection;
+//  * eigenvectors are formed as Z = Q_SBR (Q_BC Z_T) by applying the
+//    reflectors to Z_T directly (4 n^3 flops) instead of forming Q and a
+//    final GEMM (6 n^3 in the reference);
+//  * every buffer comes from the zero-initialised context arena.
+#include <cfloat>
+
+#include "common.hpp"
+
+namespace evd {
+
+int sy2sb(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, long ldw, float *Y, long ldy,
+          long wcols);
+size_t sy2sb_workspace(long n, long b, long nb);
+int band_pack(evd_ctx *ctx, long n, long b, const float *A, long lda, double *band, long ldband);
+int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E);
+int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz);
+size_t stedc_workspace(long n);
+int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
+int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
+             long ldy, long wcols);
+
+static long wy_cols(long n) { return (n + 1023) / 1024 * 1024; }
+
+// ------------------------------------------------------------- bisection
+// i-th smallest eigenvalue of the symmetric tridiagonal (D, E) by Sturm
+// counts (negative pivots of the LDL^T factorisation of T - x I).
+__global__ void sturm_bisect_kernel(int n, const double *__restrict__ D, const double *__restrict__ E, double gl,
+                                    double gu, double pivmin, double *__restrict__ w) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double lo = gl, hi = gu;
+    for (int it = 0; it < 128; it++) {
+      const double x = 0.5 * (lo + hi);
+      if (x <= lo || x >= hi) break;
+      int cnt = 0;
+      double q = D[0] - x;
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += (q < 0.0);
+      for (int k = 1; k < n; k++) {
+        const double e = E[k - 1];
+        q = (D[k] - x) - e * e / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += (q < 0.0);
+      }
+      if (cnt > i)
+        hi = x;
+      else
+        lo = x;
+    }
+    w[i] = 0.5 * (lo + hi);
+  }
+}
+
+static int sterf_bisect(evd_ctx *ctx, long n, const double *D, const double *E, double *w) {
+  std::vector<double> hD(n), hE(std::max<long>(n - 1, 1));
+  EVD_HIP(ctx, hipMemcpyAsync(hD.data(), D, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (n > 1) EVD_HIP(ctx, hipMemcpyAsync(hE.data(), E, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, ctx->stream));
+  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
+  for (long i = 0; i < n; i++) {
+    double r = (i > 0 ? fabs(hE[i - 1]) : 0.0) + (i < n - 1 ? fabs(hE[i]) : 0.0);
+    gl = std::min(gl, hD[i] - r);
+    gu = std::max(gu, hD[i] + r);
+    if (i < n - 1) emax2 = std::max(emax2, hE[i] * hE[i]);
+  }
+  const double span = std::max(gu - gl, DBL_MIN);
+  gl -= 2.0 * DBL_EPSILON * span + DBL_MIN;
+  gu += 2.0 * DBL_EPSILON * span + DBL_MIN;
+  const double pivmin = std::max(DBL_MIN, DBL_MIN * emax2);
+  sturm_bisect_kernel<<<(unsigned)((n + 127) / 128), 128, 0, ctx->stream>>>((int)n, D, E, gl, gu, pivmin, w);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
+// ------------------------------------------------------------- helpers
+__global__ void f64_to_f32_kernel(long rows, long cols, const double *__restrict__ s, long lds, float *__restrict__ d,
+                                  long ldd) {
+  for (long c = blockIdx.y; c < cols; c += gridDim.y)
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < rows; r += (long)gridDim.x * blockDim.x)
+      d[r + c * ldd] = (float)s[r + c * lds];
+}
+__global__ void f32_to_f64_kernel(long rows, long cols, const float *__restrict__ s, long lds, double *__restrict__ d,
+                                  long ldd) {
+  for (long c = blockIdx.y; c < cols; c += gridDim.y)
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < rows; r += (long)gridDim.x * blockDim.x)
+      d[r + c * ldd] = (double)s[r + c * lds];
+}
+static dim3 grid2d(long rows, long cols) {
+  return dim3((unsigned)std::min<long>((rows + 255) / 256, 64), (unsigned)std::min<long>(cols, 8192));
+}
+
+// symmetric fill of the upper triangle from the lower one + non-finite scan
+__global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda, unsigned *__restrict__ err) {
+  for (long c = blockIdx.y; c < n; c += gridDim.y)
+    for (long r = c + blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+      const float v = A[r + c * lda];
+      if (!isfinite(v)) atomicOr(err, (unsigned)EVD_E_NONFINITE);
+      A[c + r * lda] = v;
+    }
+}
+
+__global__ void copy_f64_kernel(long n, const double *__restrict__ s, double *__restrict__ d) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+size_t syevd_workspace(long n, long b, long nb, char jobz) {
+  const long wc = wy_cols(n);
+  size_t s = 0;
+  s += align256((size_t)n * wc * sizeof(float)) * 2;          // W, Y
+  s += align256((size_t)2 * b * n * sizeof(double));          // band
+  s += align256((size_t)(n + b) * n * sizeof(float));         // U
+  s += align256((size_t)n * sizeof(double)) * 2;              // D, E
+  size_t later = sy2sb_workspace(n, b, nb);
+  if (jobz == 'V' || jobz == 'v') {
+    s += align256((size_t)n * n * sizeof(double));  // Z_T
+    size_t dc = stedc_workspace(n);
+    size_t back = align256((size_t)1024 * n * sizeof(float)) + align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
+                  ((size_t)64 << 20);
+    later = std::max(later, std::max(dc, back));
+  }
+  return s + later + ((size_t)4 << 20);
+}
+
+static void stage_mark(evd_ctx *ctx, int i) {
+  if (ctx->timing) hipEventRecord(ctx->ev[i], ctx->stream);
+}
+
+int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz) {
+  const bool vec = (jobz == 'V' || jobz == 'v');
+  if (n == 0) return 0;
+  ArenaFrame frame(ctx);
+  const long wc = wy_cols(n);
+  float *W = arena_alloc_n<float>(ctx, (size_t)n * wc);
+  float *Y = arena_alloc_n<float>(ctx, (size_t)n * wc);
+  double *band = arena_alloc_n<double>(ctx, (size_t)2 * b * n);
+  const long ldu = n + b;
+  float *U = arena_alloc_n<float>(ctx, (size_t)ldu * n);
+  double *D = arena_alloc_n<double>(ctx, (size_t)n);
+  double *E = arena_alloc_n<double>(ctx, (size_t)std::max<long>(n, 1));
+  double *Zt = vec ? arena_alloc_n<double>(ctx, (size_t)n * n) : nullptr;
+  if (!W || !Y || !band || !U || !D || !E || (vec && !Zt)) return EVD_E_NOMEM;
+
+  // full symmetric storage from the lower triangle, NaN/Inf screen
+  symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A, lda, ctx->ctrl + kCtrlError);
+  EVD_LAUNCHED(ctx);
+  EVD_TRY(check_device_error(ctx, "input screen"));
+
+  stage_mark(ctx, 0);
+  EVD_TRY(sy2sb(ctx, n, b, nb, A, lda, W, n, Y, n, wc));
+  EVD_TRY(band_pack(ctx, n, b, A, lda, band, 2 * b));
+  stage_mark(ctx, 1);
+  EVD_TRY(sb2st(ctx, n, b, band, 2 * b, U, ldu, D, E));
+  stage_mark(ctx, 2);
+  if (vec) {
+    EVD_TRY(stedc(ctx, n, D, E, Zt, n));
+    copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D, w);
+    EVD_LAUNCHED(ctx);
+    f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt, n, Z, ldz);
+    EVD_LAUNCHED(ctx);
+    stage_mark(ctx, 3);
+    EVD_TRY(bc_back(ctx, n, b, n, Z, ldz, U, ldu, true));
+    stage_mark(ctx, 4);
+    EVD_TRY(sbr_back(ctx, n, b, n, Z, ldz, W, n, Y, n, wc));
+    stage_mark(ctx, 5);
+  } else {
+    EVD_TRY(sterf_bisect(ctx, n, D, E, w));
+    stage_mark(ctx, 3);
+    stage_mark(ctx, 4);
+    stage_mark(ctx, 5);
+  }
+  EVD_TRY(check_device_error(ctx, "evd_syevd_f32"));
+  if (ctx->timing) {
+    for (int i = 0; i < EVD_STAGE_COUNT; i++) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]);
+      ctx->stage_ms[i] = ms;
+    }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------- metrics
+__global__ void frob_sq_kernel(long m, long n, const double *__restrict__ X, long ldx, int minus_identity,
+                               double *__restrict__ out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (long c = blockIdx.y; c < n; c += gridDim.y)
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < m; r += (long)gridDim.x * blockDim.x) {
+      double v = X[r + c * ldx];
+      if (minus_identity && r == c) v -= 1.0;
+      s += v * v;
+    }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+}
+
+__global__ void scale_cols_sub_kernel(long n, double *__restrict__ R, long ldr, const double *__restrict__ Z, long ldz,
+                                      const double *__restrict__ w) {
+  for (long c = blockIdx.y; c < n; c += gridDim.y)
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+      R[r + c * ldr] -= Z[r + c * ldz] * w[c];
+}
+
+static int frob_sq(evd_ctx *ctx, long m, long n, const double *X, long ldx, bool minus_identity, double *dacc,
+                   double *host) {
+  double zero = 0.0;
+  EVD_HIP(ctx, hipMemcpyAsync(dacc, &zero, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  frob_sq_kernel<<<grid2d(m, n), 256, 0, ctx->stream>>>(m, n, X, ldx, minus_identity ? 1 : 0, dacc);
+  EVD_LAUNCHED(ctx);
+  EVD_HIP(ctx, hipMemcpyAsync(host, dacc, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // namespace evd
+
+using namespace evd;
+
+extern "C" size_t evd_syevd_workspace_f32(long n, long b, long nb, char jobz) {
+  if (n <= 0) return 0;
+  return syevd_workspace(n, b, nb, jobz);
+}
+
+extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z,
+                             long ldz) {
+  if (!ctx) return -1;
+  const bool vec = (jobz == 'V' || jobz == 'v');
+  EVD_ARG(ctx, vec || jobz == 'N' || jobz == 'n', 2);
+  EVD_ARG(ctx, n >= 0, 3);
+  EVD_ARG(ctx, b == 32 || b == 64 || (!vec && b >= 1 && b <= 64), 4);
+  EVD_ARG(ctx, nb >= b && nb % b == 0, 5);
+  EVD_ARG(ctx, A != nullptr || n == 0, 6);
+  EVD_ARG(ctx, lda >= std::max<long>(1, n), 7);
+  EVD_ARG(ctx, w != nullptr || n == 0, 8);
+  EVD_ARG(ctx, !vec || Z != nullptr || n == 0, 9);
+  EVD_ARG(ctx, !vec || ldz >= std::max<long>(1, n), 10);
+  if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_f32 cannot nest");
+  EVD_TRY(arena_ensure(ctx, syevd_workspace(n, b, nb, jobz)));
+  return syevd(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+}
+
+extern "C" int evd_orthogonality_f32(evd_ctx *ctx, long m, long n, const float *Q, long ldq, double *out) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, m >= 0, 2);
+  EVD_ARG(ctx, n >= 0, 3);
+  EVD_ARG(ctx, ldq >= std::max<long>(1, m), 5);
+  EVD_ARG(ctx, out != nullptr, 6);
+  if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "metrics cannot nest");
+  EVD_TRY(arena_ensure(ctx, align256((size_t)m * n * 8) + align256((size_t)n * n * 8) + 4096));
+  ArenaFrame frame(ctx);
+  double *Q64 = arena_alloc_n<double>(ctx, (size_t)m * n);
+  double *G = arena_alloc_n<double>(ctx, (size_t)n * n);
+  double *acc = arena_alloc_n<double>(ctx, 1);
+  if (!Q64 || !G || !acc) return EVD_E_NOMEM;
+  f32_to_f64_kernel<<<grid2d(m, n), 256, 0, ctx->stream>>>(m, n, Q, ldq, Q64, m);
+  EVD_LAUNCHED(ctx);
+  EVD_TRY(dgemm(ctx, 'T', 'N', n, n, m, 1.0, Q64, m, Q64, m, 0.0, G, n));
+  double s = 0.0;
+  EVD_TRY(frob_sq(ctx, n, n, G, n, true, acc, &s));
+  *out = sqrt(s) / (double)(n > 0 ? n : 1);
+  return 0;
+}
+
+extern "C" int evd_backward_error_f32(evd_ctx *ctx, long n, const float *A, long lda, const double *w, const float *Z,
+                                      long ldz, double *out) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, n >= 0, 2);
+  EVD_ARG(ctx, lda >= std::max<long>(1, n), 4);
+  EVD_ARG(ctx, ldz >= std::max<long>(1, n), 7);
+  EVD_ARG(ctx, out != nullptr, 8);
+  if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "metrics cannot nest");
+  EVD_TRY(arena_ensure(ctx, 3 * align256((size_t)n * n * 8) + 4096));
+  ArenaFrame frame(ctx);
+  double *A64 = arena_alloc_n<double>(ctx, (size_t)n * n);
+  double *Z64 = arena_alloc_n<double>(ctx, (size_t)n * n);
+  double *R = arena_alloc_n<double>(ctx, (size_t)n * n);
+  double *acc = arena_alloc_n<double>(ctx, 1);
+  if (!A64 || !Z64 || !R || !acc) return EVD_E_NOMEM;
+  f32_to_f64_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, A, lda, A64, n);
+  EVD_LAUNCHED(ctx);
+  f32_to_f64_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Z, ldz, Z64, n);
+  EVD_LAUNCHED(ctx);
+  EVD_TRY(dgemm(ctx, 'N', 'N', n, n, n, 1.0, A64, n, Z64, n, 0.0, R, n));
+  scale_cols_sub_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, R, n, Z64, n, w);
+  EVD_LAUNCHED(ctx);
+  double rr = 0.0, aa = 0.0;
+  EVD_TRY(frob_sq(ctx, n, n, R, n, false, acc, &rr));
+  EVD_TRY(frob_sq(ctx, n, n, A64, n, false, acc, &aa));
+  *out = (aa > 0.0) ? sqrt(rr) / ((double)n * sqrt(aa)) : sqrt(rr);
+  return 0;
+}

@@ -1,0 +1,191 @@
+/*
+ * evd4sc.h - C ABI of the MI355X two-stage symmetric eigensolver.
+ *
+ * Every entry point takes plain device pointers (column-major, 64-bit leading
+ * dimensions), runs stream-ordered on the context's HIP stream, and returns an
+ * int status: 0 = success, -k = argument k is invalid (LAPACK style),
+ * positive = runtime / numerical failure (see EVD_E_* below).  No entry point
+ * exits the process; the message of the last failure is available from
+ * evd_ctx_last_error().  Scratch memory comes from the context's arena, which
+ * the library zero-initialises where an algorithm needs zeros; callers never
+ * rely on hipMalloc returning zeroed memory.
+ *
+ * Each function cites the reference interface it replaces (file:line in
+ * hansheng1001/EVD4SC2025).  The reference passes cuBLAS/cuSOLVER handles; here
+ * an evd_ctx carries the stream, the arena and the tuning.
+ */
+#ifndef EVD4SC_H
+#define EVD4SC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct evd_ctx evd_ctx;
+
+/* status codes (positive) */
+#define EVD_OK 0
+#define EVD_E_HIP 1        /* a HIP runtime call failed */
+#define EVD_E_NOMEM 2      /* workspace / HBM budget exceeded */
+#define EVD_E_NONFINITE 3  /* input held NaN or Inf */
+#define EVD_E_TIMEOUT 4    /* a persistent kernel's bounded wait expired */
+#define EVD_E_UNSUPPORTED 5
+#define EVD_E_NOCONVERGE 6 /* secular equation did not converge */
+
+/* precision of the GEMM work in SBR and the back transformation */
+#define EVD_PREC_FP32 0 /* plain FP32 MFMA / FP32 VALU */
+#define EVD_PREC_BF16 1 /* BF16 MFMA, FP32 accumulate, hi/lo split */
+#define EVD_PREC_FP16 2 /* FP16 MFMA, FP32 accumulate, hi/lo split */
+
+/* stage indices of evd_ctx_stage_ms() */
+#define EVD_STAGE_SBR 0
+#define EVD_STAGE_BC 1
+#define EVD_STAGE_DC 2
+#define EVD_STAGE_BCBACK 3
+#define EVD_STAGE_SBRBACK 4
+#define EVD_STAGE_COUNT 5
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Create a context on `device`.  `stream` is a hipStream_t (NULL = the
+ * context creates its own non-blocking stream).  Replaces the handle creation
+ * of src/EVD/myEVD.cu:272-276. */
+int evd_ctx_create(int device, void *stream, evd_ctx **out);
+int evd_ctx_destroy(evd_ctx *ctx);
+int evd_ctx_set_stream(evd_ctx *ctx, void *stream);
+void *evd_ctx_get_stream(evd_ctx *ctx);
+const char *evd_ctx_last_error(const evd_ctx *ctx);
+/* Grow the arena to at least `bytes` now (outside any timed region). */
+int evd_ctx_reserve(evd_ctx *ctx, size_t bytes);
+size_t evd_ctx_arena_bytes(const evd_ctx *ctx);
+/* Refuse any arena growth beyond `bytes` (0 = unlimited). */
+int evd_ctx_set_budget(evd_ctx *ctx, size_t bytes);
+/* Tuning: panel/bandwidth are arguments of the calls; these are the knobs of
+ * the kernels (0 = library default).  bc_sweeps = sweeps per BC-back block,
+ * bc_waves = pipelined waves per BC-back workgroup, chase_groups = persistent
+ * workgroups of the bulge chase, sbr_back_block = WY block width of the SBR
+ * back transformation. */
+int evd_ctx_set_tuning(evd_ctx *ctx, int bc_sweeps, int bc_waves, int chase_groups, int sbr_back_block);
+int evd_ctx_set_precision(evd_ctx *ctx, int precision);
+/* Per-stage device time (ms) of the last evd_syevd_f32 call, measured with
+ * hipEvents on the context stream; enable with evd_ctx_enable_timing. */
+int evd_ctx_enable_timing(evd_ctx *ctx, int on);
+int evd_ctx_stage_ms(const evd_ctx *ctx, float *ms, int count);
+int evd_ctx_synchronize(evd_ctx *ctx);
+
+/* ---- building blocks with reference counterparts ------------------------ */
+
+/* Tall-skinny Householder panel QR of the m x nb panel A (nb <= 64):
+ * A = (I - W Y^T) [R; 0], Y unit lower (LAPACK V), W = V T (dlarft forward
+ * columnwise), R upper.  On exit A holds Y (as the reference's panelQR does),
+ * W and R are written.  Reflector generation runs in FP64.
+ * Replaces panelQR<T> (api/panelQR.cu:20-246, decl api/inc/PanelQR.h:7-19)
+ * and hou_tsqr_panel (api/inc/TallShinnyQR.h:9-134). */
+int evd_panel_qr_f32(evd_ctx *ctx, long m, long nb, float *A, long lda, float *W, long ldw, float *R, long ldr);
+
+/* Symmetric rank-2k update of the lower triangle, mirrored to the upper:
+ * C = alpha (A B^T + B A^T) + beta C, A and B n x k.
+ * Replaces tc_ozimmu_syr2k<T> (api/TCblas/tc_ozimmu_syr2k.cu:209-341,
+ * decl api/inc/tc_ozimmu_syr2k.h:6-18). */
+int evd_syr2k_f32(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
+                  float beta, float *C, long ldc);
+
+/* Full -> band reduction (SBR) with ZY two-sided updates.  A (n x n,
+ * symmetric, full storage) is overwritten: its lower band of bandwidth b is
+ * B = Q^T A Q, where Q = prod_p (I - W_p Y_p^T); W and Y (n x n) receive all
+ * panels (panel p at row (p+1)b, column pb; zeros elsewhere).
+ * Replaces my_ZY_ZY_SBR_Vector<T> (api/zy_zy_sy2sb.cu:24-486,
+ * decl api/inc/zy_zy_sy2sb.h:5-25); the reference's dOriA/dZ/dR/work scratch
+ * comes from the arena. */
+int evd_sy2sb_f32(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, long ldw, float *Y, long ldy);
+
+/* Dense lower band (bandwidth b) of A -> FP64 band storage
+ * band[(r-c) + c*ldband], 0 <= r-c < 2b, rows b+1..2b-1 zeroed.
+ * Replaces kernel_bugle_chasing_cpydA2dSubA (src/EVD/myEVD.cu:53-78). */
+int evd_band_pack_f32(evd_ctx *ctx, long n, long b, const float *A, long lda, double *band, long ldband);
+
+/* Band -> tridiagonal bulge chasing.  band as produced by evd_band_pack_f32
+ * is destroyed; D (n) and E (n-1) receive T's diagonal and subdiagonal.  The
+ * reflectors H = I - u u^T (||u||^2 = 2) are stored in U ((n+b) x n, ldu >=
+ * n+b): sweep s, step j at U[s+1+j*b .. , s] ("noBandU" layout).  All of U is
+ * written (zeros outside the reflectors).
+ * Replaces chasing_kernel_one_timeV10<32> and its cooperative launch
+ * (util/inc/bugle_chasingV10.h:21-392, src/EVD/myEVD.cu:365-451) plus
+ * kernel_bugle_chasing_cpydSubA2dA / cpyATr2Vector (src/EVD/myEVD.cu:105-123,
+ * util/kernelOther/kernelOther.cu:399-440). */
+int evd_sb2st_f64(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E);
+
+/* Symmetric tridiagonal divide and conquer on the GPU in FP64.  On exit D
+ * holds the eigenvalues in ascending order and Z (n x n) the orthonormal
+ * eigenvectors (compz = 'I').  E is destroyed.
+ * Replaces LAPACKE_dstedc(LAPACK_COL_MAJOR,'I',...) (src/EVD/myEVD.cu:529). */
+int evd_stedc_f64(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz);
+
+/* Merge the N/slide slide-wide WY blocks of W, Y (M x N) left to right into
+ * one W with prod_k (I - W_k Y_k^T) = I - W Y^T.
+ * Replaces computerWYFromSlide (api/computerWYFromSlide.cu:5-25). */
+int evd_wy_from_slide_f32(evd_ctx *ctx, long M, long N, long slide, float *W, long ldw, const float *Y, long ldy);
+
+/* Q = I - W Y^T (M x M).  Replaces computerQFromWY (api/computerQFromWY.cu:5-18). */
+int evd_q_from_wy_f32(evd_ctx *ctx, long M, long N, float *Q, long ldq, const float *W, long ldw, const float *Y,
+                      long ldy);
+
+/* Apply the bulge-chasing reflectors in U to the ncols columns of Q.
+ * order = 0: Q <- H_K ... H_1 Q (first-generated reflector first), which is
+ *            what my_BC_back_trans_v8_10_noBandU computes
+ *            (api/BC_backTransV8_10_noBandU.cu:234-313, decl
+ *            api/inc/BC_backTrans.h:3);
+ * order = 1: Q <- H_1 ... H_K Q = Q_BC Q (eigenvectors of the band from those
+ *            of T).  b must be 32 or 64. */
+int evd_bc_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, int order);
+
+/* Apply Q_SBR = prod_p (I - W_p Y_p^T) from the left to the ncols columns of
+ * Z: Z <- Q_SBR Z.  W is overwritten by the merged blocks (as the reference's
+ * doubling merge does).  Replaces the SBR part of my_SBRBack_BCBack
+ * (src/EVD/myEVD.cu:135-245), applied to Z directly instead of forming Q. */
+int evd_sbr_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
+                     long ldy);
+
+/* Whole pipeline: eigenvalues w (FP64, ascending, n) and, if jobz == 'V',
+ * eigenvectors Z (n x n FP32) of the symmetric A (lower triangle read; A is
+ * destroyed).  b = bandwidth (32 or 64), nb = SBR block.  Replaces main's
+ * SBR -> BC -> D&C -> back-transformation sequence (src/EVD/myEVD.cu:251-571). */
+int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz);
+/* Arena bytes evd_syevd_f32 needs for (n, b, nb, jobz). */
+size_t evd_syevd_workspace_f32(long n, long b, long nb, char jobz);
+
+/* ---- metrics (util/tool/checkMetric.cu:5-25, src/EVD/myEVD.cu:598-639) ---- */
+
+/* ||Q^T Q - I||_F / n, computed in FP64 on the GPU. */
+int evd_orthogonality_f32(evd_ctx *ctx, long m, long n, const float *Q, long ldq, double *out);
+/* ||A Z - Z diag(w)||_F / (n ||A||_F), FP64 on the GPU (A full symmetric). */
+int evd_backward_error_f32(evd_ctx *ctx, long n, const float *A, long lda, const double *w, const float *Z, long ldz,
+                           double *out);
+
+/* ---- dense kernels exposed for tests ------------------------------------ */
+
+/* C = alpha op(A) op(B) + beta C on MFMA (FP32 in/out, FP32 accumulate).
+ * ta, tb in {'N','T'}. */
+int evd_sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, const float *A, long lda,
+              const float *B, long ldb, float beta, float *C, long ldc);
+/* Same in FP64 on the FP64 MFMA path. */
+int evd_dgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, double alpha, const double *A, long lda,
+              const double *B, long ldb, double beta, double *C, long ldc);
+
+/* ---- synthetic inputs (device-side, seeded) ------------------------------ */
+
+/* A = (X + X^T)/2 with X i.i.d. uniform[-1,1] (kind 0) or N(0,1) (kind 1)
+ * from a counter-based Philox4x32-10 stream keyed by seed. */
+int evd_gen_symmetric_f32(evd_ctx *ctx, long n, unsigned long long seed, int kind, float *A, long lda);
+/* A = Q diag(lam) Q^T with Q the product of n Householder reflectors whose
+ * vectors are N(0, I) from the same Philox stream (stream 1); built in FP64
+ * on the GPU, rounded to FP32.  lam: device array of n doubles. */
+int evd_gen_spectrum_f32(evd_ctx *ctx, long n, unsigned long long seed, const double *lam, float *A, long lda);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EVD4SC_H */

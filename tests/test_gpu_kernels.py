@@ -1,0 +1,219 @@
+"""GPU parity of the building blocks against the oracle / FP64 references,
+through the C ABI (libevd4sc.so)."""
+import numpy as np
+import pytest
+
+from oracle import evd_oracle as O
+from tests.helpers import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(300, 200, 100), (4000, 32, 3000), (32, 32, 5000), (64, 500, 70), (1, 7, 3)])
+def test_sgemm_matches_fp64(ctx, ta, tb, m, n, k):
+    import evd4sc2025_amd as E
+
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n)).astype(np.float32)
+    C0 = rng.standard_normal((m, n)).astype(np.float32)
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C0)
+    E.sgemm(ctx, ta, tb, 0.75, dA, dB, -0.5, dC)
+    opA = A.T if ta == "T" else A
+    opB = B.T if tb == "T" else B
+    want = 0.75 * (opA.astype(np.float64) @ opB.astype(np.float64)) - 0.5 * C0
+    got = to_host(dC)
+    err = np.abs(got - want).max() / max(1.0, np.abs(want).max())
+    assert err < 5e-6 * np.sqrt(k), err
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_dgemm_matches_fp64(ctx, ta, tb):
+    import evd4sc2025_amd as E
+
+    rng = np.random.default_rng(5)
+    m, n, k = 130, 70, 90
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    C0 = rng.standard_normal((m, n))
+    dC = to_dev(C0)
+    E.dgemm(ctx, ta, tb, 1.5, to_dev(A), to_dev(B), 0.25, dC)
+    want = 1.5 * ((A.T if ta == "T" else A) @ (B.T if tb == "T" else B)) + 0.25 * C0
+    np.testing.assert_allclose(to_host(dC), want, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("n,k", [(200, 64), (1000, 257), (129, 1)])
+def test_syr2k_lower_mirror(ctx, n, k):
+    import evd4sc2025_amd as E
+
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, k)).astype(np.float32)
+    B = rng.standard_normal((n, k)).astype(np.float32)
+    C = rng.standard_normal((n, n)).astype(np.float32)
+    C = (C + C.T) / 2
+    dC = to_dev(C)
+    E.tc_ozimmu_syr2k(ctx, -1.0, to_dev(A), to_dev(B), 1.0, dC)
+    got = to_host(dC)
+    want = O.syr2k(-1.0, A.astype(np.float64), B.astype(np.float64), 1.0, C.astype(np.float64))
+    assert np.array_equal(got, got.T)
+    assert np.abs(got - want).max() < 1e-5 * np.sqrt(k) * max(1.0, np.abs(want).max())
+
+
+@pytest.mark.parametrize("n,seed", [(64, 0), (130, 1), (333, 7)])
+def test_generator_bit_exact(ctx, n, seed):
+    import evd4sc2025_amd as E
+
+    A = to_host(E.gen_symmetric(ctx, n, seed, 0))
+    assert np.array_equal(A, O.gen_symmetric(n, seed, 0))
+
+
+@pytest.mark.parametrize("m,b", [(64, 32), (200, 32), (1000, 32), (5000, 32), (160, 64), (3000, 64), (20, 32)])
+def test_panel_qr_matches_oracle(ctx, m, b):
+    import evd4sc2025_amd as E
+
+    torch = _torch()
+    P = O.gen_symmetric(max(m, b), 40 + m)[:m, :b]
+    V, T, R, tau = O.panel_qr(P.astype(np.float64))
+    k = V.shape[1]
+    dA = to_dev(P)
+    dW = E.colmajor(m, b, fill=0.0)
+    dR = E.colmajor(b, b, fill=0.0)
+    E.panelQR(ctx, dA, dW, dR)
+    torch.cuda.synchronize()
+    Y = to_host(dA)
+    scale = max(1.0, np.abs(P).max())
+    np.testing.assert_allclose(Y[:, :k], V, atol=2e-5 * np.sqrt(m))
+    np.testing.assert_allclose(to_host(dW)[:, :k], V @ T, atol=2e-5 * np.sqrt(m))
+    kr = min(b, m)
+    np.testing.assert_allclose(np.triu(to_host(dR))[:kr], R[:kr], atol=2e-5 * scale * np.sqrt(m))
+    # the factorisation reproduces the panel: (I - W Y^T)[R;0] = P
+    W = to_host(dW).astype(np.float64)
+    Rf = np.zeros((m, b))
+    Rf[:kr] = np.triu(to_host(dR))[:kr]
+    rec = Rf - W @ (Y.astype(np.float64).T @ Rf)
+    assert np.abs(rec - P).max() < 1e-5 * scale * np.sqrt(m)
+
+
+@pytest.mark.parametrize("n,b,nb", [(64, 32, 32), (130, 32, 64), (512, 32, 128), (700, 64, 128), (1000, 32, 1024)])
+def test_sy2sb_band_spectrum(ctx, n, b, nb):
+    import evd4sc2025_amd as E
+
+    A = O.gen_symmetric(n, 200 + n)
+    dA = to_dev(A)
+    _, dW, dY = E.sy2sb(ctx, dA, b, nb)
+    Ab = to_host(dA).astype(np.float64)
+    B = O.band_of(Ab, b)
+    wA = np.linalg.eigvalsh(A.astype(np.float64))
+    wB = np.linalg.eigvalsh(B)
+    nrm = np.abs(wA).max()
+    assert np.abs(wA - wB).max() < 2e-6 * nrm * max(1.0, np.sqrt(n / 512))
+    # Q_SBR^T A Q_SBR = B with Q from the returned W, Y
+    Q = O.q_sbr(to_host(dW).astype(np.float64), to_host(dY).astype(np.float64), b)
+    assert O.orthogonality(Q) < 1e-6
+    assert np.abs(Q.T @ A.astype(np.float64) @ Q - B).max() < 1e-5 * nrm
+
+
+@pytest.mark.parametrize("n,b", [(40, 32), (130, 32), (600, 32), (300, 64), (1500, 32)])
+def test_sb2st_matches_oracle(ctx, n, b):
+    import evd4sc2025_amd as E
+
+    A = O.gen_symmetric(n, 300 + n).astype(np.float64)
+    Ab, _, _ = O.sy2sb(A, b, 2 * b)
+    B = O.band_of(Ab, b)
+    D0, E0, U0 = O.sb2st(B, b)
+    band = E.band_pack(ctx, to_dev(B.astype(np.float32)), b)
+    D, Ee, U = E.sb2st(ctx, band, n, b)
+    D, Ee, U = to_host(D), to_host(Ee), to_host(U)
+    # same algorithm in FP64 on the same (FP32-rounded) band
+    B32 = B.astype(np.float32).astype(np.float64)
+    D1, E1, U1 = O.sb2st(B32, b)
+    np.testing.assert_allclose(D, D1, atol=1e-10)
+    np.testing.assert_allclose(Ee, E1, atol=1e-10)
+    np.testing.assert_allclose(U, U1, atol=1e-5)
+    wT = np.sort(np.linalg.eigvalsh(np.diag(D) + np.diag(Ee, 1) + np.diag(Ee, -1)))
+    np.testing.assert_allclose(wT, np.linalg.eigvalsh(B32), atol=1e-10)
+
+
+def _tridiag_cases():
+    rng = np.random.default_rng(17)
+    cases = []
+    for n in [1, 2, 3, 17, 100, 513, 2000]:
+        cases.append((f"rand{n}", rng.uniform(-1, 1, n), rng.uniform(-1, 1, max(n - 1, 0))))
+    k = 10
+    cases.append(("wilkinson21", np.abs(np.arange(-k, k + 1)).astype(float), np.ones(2 * k)))
+    cases.append(("diag", rng.uniform(-1, 1, 50), np.zeros(49)))
+    cases.append(("toeplitz", 2 * np.ones(300), np.ones(299)))
+    d = np.repeat(rng.uniform(-1, 1, 8), 64) + 1e-9 * rng.standard_normal(512)
+    cases.append(("cluster", d, 1e-7 * rng.standard_normal(511)))
+    cases.append(("glued", np.ones(400), np.where(np.arange(399) % 50 == 49, 1e-12, 1.0)))
+    return cases
+
+
+@pytest.mark.parametrize("name,d,e", _tridiag_cases(), ids=[c[0] for c in _tridiag_cases()])
+def test_stedc_matches_lapack(ctx, name, d, e):
+    import evd4sc2025_amd as E
+
+    n = d.size
+    w, Z = E.stedc(ctx, to_dev(d), to_dev(e))
+    w, Z = to_host(w), to_host(Z)
+    w0 = O.tridiag_eig(d, e, vectors=False)
+    scale = max(1.0, np.abs(w0).max())
+    np.testing.assert_allclose(w, w0, atol=1e-13 * scale * max(1, n) ** 0.5)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    resid = np.linalg.norm(T @ Z - Z * w[None, :]) / (scale * max(n, 1))
+    assert resid < 1e-14, resid
+    assert O.orthogonality(Z) < 1e-14
+
+
+@pytest.mark.parametrize("n,b,order", [(70, 32, 1), (70, 32, 0), (300, 32, 1), (300, 32, 0), (257, 64, 1),
+                                       (257, 64, 0), (1100, 32, 1)])
+def test_bc_back_matches_oracle(ctx, n, b, order):
+    import evd4sc2025_amd as E
+
+    A = O.gen_symmetric(n, 400 + n).astype(np.float64)
+    Ab, _, _ = O.sy2sb(A, b, 2 * b)
+    _, _, U = O.sb2st(O.band_of(Ab, b), b)
+    U32 = U.astype(np.float32)
+    X = np.random.default_rng(n).standard_normal((n, 150)).astype(np.float32)
+    dX = to_dev(X)
+    dU = to_dev(U32)
+    E.bc_back(ctx, dX, dU, n, b, order)
+    want = O.bc_back(X.astype(np.float64), U32.astype(np.float64), n, b, order)
+    assert np.abs(to_host(dX) - want).max() < 2e-5 * np.abs(X).max()
+
+
+@pytest.mark.parametrize("n,b", [(256, 32), (1024, 32), (700, 64)])
+def test_sbr_back_and_wy(ctx, n, b):
+    import evd4sc2025_amd as E
+
+    A = O.gen_symmetric(n, 500 + n)
+    dA = to_dev(A)
+    _, dW, dY = E.sy2sb(ctx, dA, b, 2 * b)
+    W, Y = to_host(dW).astype(np.float64), to_host(dY).astype(np.float64)
+    Q = O.q_sbr(W, Y, b)
+    X = np.random.default_rng(1).standard_normal((n, 64)).astype(np.float32)
+    dX = to_dev(X)
+    E.sbr_back(ctx, dX, dW, dY, b)
+    assert np.abs(to_host(dX) - Q @ X).max() < 2e-5 * np.sqrt(n / 256) * np.abs(X).max()
+
+
+def test_wy_from_slide_and_q_from_wy(ctx):
+    import evd4sc2025_amd as E
+
+    rng = np.random.default_rng(3)
+    M, slide, nblk = 300, 32, 6
+    W = (0.1 * rng.standard_normal((M, slide * nblk))).astype(np.float32)
+    Y = (0.1 * rng.standard_normal((M, slide * nblk))).astype(np.float32)
+    want = O.q_from_wy(O.wy_from_slide(W.astype(np.float64), Y.astype(np.float64), slide), Y.astype(np.float64))
+    dW, dY = to_dev(W), to_dev(Y)
+    E.computerWYFromSlide(ctx, slide, dW, dY)
+    dQ = E.colmajor(M, M)
+    E.computerQFromWY(ctx, dQ, dW, dY)
+    assert np.abs(to_host(dQ) - want).max() < 1e-4

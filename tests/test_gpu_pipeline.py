@@ -1,0 +1,109 @@
+"""End-to-end parity of evd_syevd_f32 (SBR -> BC -> D&C -> back
+transformation) against FP64 LAPACK dsyevd on the same seeded inputs.
+
+Tolerances (north-star): eigenvalues within 1e-5 ||A||_2; backward error
+||AZ - Z diag(w)||_F / (n ||A||_F) and orthogonality ||Z^T Z - I||_F / n at
+most 1e-6."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import evd_oracle as O
+from tests.helpers import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+G = np.load(Path(__file__).parent / "golden" / "golden.npz")
+
+
+@pytest.mark.parametrize("n,seed,b,nb", [(64, 0, 32, 32), (130, 1, 32, 64), (512, 2, 32, 128), (2048, 0, 32, 1024)])
+def test_syevd_matches_golden_dsyevd(ctx, n, seed, b, nb):
+    import evd4sc2025_amd as E
+
+    dA = E.gen_symmetric(ctx, n, seed, 0)
+    A = to_host(dA)
+    r = E.syevd(ctx, dA, b=b, nb=nb, jobz="V")
+    w = to_host(r.w)
+    w0 = G[f"syevd_w_n{n}_s{seed}"]
+    a2 = np.abs(w0).max()
+    assert np.abs(w - w0).max() <= 1e-5 * a2
+    be = E.backward_error(ctx, dA, r.w, r.Z)
+    orth = E.checkOrthogonality(ctx, r.Z)
+    assert be <= 1e-6 and orth <= 1e-6, (be, orth)
+    # cross-check the GPU metric against the host definition
+    if n <= 512:
+        Z = to_host(r.Z).astype(np.float64)
+        assert abs(O.backward_error(A, w, Z) - be) <= 1e-3 * be + 1e-15
+        assert abs(O.orthogonality(Z) - orth) <= 1e-3 * orth + 1e-15
+
+
+@pytest.mark.parametrize("n,b,nb", [(97, 32, 64), (300, 64, 128), (1000, 32, 256), (777, 64, 64), (33, 32, 32),
+                                    (2, 32, 32), (1, 32, 32)])
+def test_syevd_ragged_sizes(ctx, n, b, nb):
+    import evd4sc2025_amd as E
+
+    dA = E.gen_symmetric(ctx, n, 600 + n, 1)
+    A = to_host(dA).astype(np.float64)
+    r = E.syevd(ctx, dA, b=b, nb=nb)
+    w0 = O.dsyevd(A, vectors=False)
+    a2 = max(np.abs(w0).max(), 1e-30)
+    assert np.abs(to_host(r.w) - w0).max() <= 1e-5 * a2
+    if n > 1:
+        assert E.backward_error(ctx, dA, r.w, r.Z) <= 1e-6
+        assert E.checkOrthogonality(ctx, r.Z) <= 1e-6
+
+
+@pytest.mark.parametrize("n", [256, 1000])
+def test_syevd_eigenvalues_only(ctx, n):
+    import evd4sc2025_amd as E
+
+    dA = E.gen_symmetric(ctx, n, 700 + n, 0)
+    A = to_host(dA).astype(np.float64)
+    r = E.syevd(ctx, dA, b=32, nb=128, jobz="N")
+    w0 = O.dsyevd(A, vectors=False)
+    assert r.Z is None
+    assert np.abs(to_host(r.w) - w0).max() <= 1e-5 * np.abs(w0).max()
+
+
+def test_syevd_known_spectrum(ctx):
+    """A = Q diag(lambda) Q^T with lambda_i = +-10^(-6 i/(n-1)): the exact
+    eigenvalues are known in closed form."""
+    import torch
+
+    import evd4sc2025_amd as E
+
+    n = 1024
+    rng = np.random.default_rng(0)
+    lam = 10.0 ** (-6.0 * np.arange(n) / (n - 1)) * np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    dlam = torch.from_numpy(lam).cuda()
+    dA = E.gen_spectrum(ctx, dlam, 5)
+    A = to_host(dA).astype(np.float64)
+    np.testing.assert_allclose(np.linalg.eigvalsh(A), np.sort(lam), atol=1e-6)
+    r = E.syevd(ctx, dA, b=32, nb=256)
+    assert np.abs(to_host(r.w) - np.sort(lam)).max() <= 1e-5
+    assert E.backward_error(ctx, dA, r.w, r.Z) <= 1e-6
+
+
+def test_syevd_rejects_bad_arguments(ctx):
+    import evd4sc2025_amd as E
+
+    dA = E.gen_symmetric(ctx, 64, 0, 0)
+    with pytest.raises(E.EvdError) as ei:
+        E.syevd(ctx, dA, b=48, nb=96)  # b must be 32 or 64 for vectors
+    assert ei.value.code == -4
+    with pytest.raises(E.EvdError) as ei:
+        E.syevd(ctx, dA, b=32, nb=48)  # nb % b != 0
+    assert ei.value.code == -5
+
+
+def test_syevd_nonfinite_input_is_reported(ctx):
+    import evd4sc2025_amd as E
+
+    dA = E.gen_symmetric(ctx, 64, 0, 0)
+    dA[3, 1] = float("nan")
+    with pytest.raises(E.EvdError) as ei:
+        E.syevd(ctx, dA, b=32, nb=32)
+    assert ei.value.code == 3
+    # the context stays usable
+    r = E.syevd(ctx, E.gen_symmetric(ctx, 64, 0, 0), b=32, nb=32)
+    assert np.isfinite(to_host(r.w)).all()
