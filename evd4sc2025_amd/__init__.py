@@ -19,7 +19,7 @@ from dataclasses import dataclass
 from ._native import EvdError, lib
 
 __all__ = [
-    "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "tc_ozimmu_syr2k", "band_pack",
+    "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "hou_tsqr_panel", "tc_ozimmu_syr2k", "band_pack",
     "sb2st", "stedc", "computerWYFromSlide", "computerQFromWY", "my_BC_back_trans_v8_10_noBandU", "bc_back",
     "sbr_back", "checkOrthogonality", "backward_error", "sgemm", "dgemm", "gen_symmetric", "gen_spectrum",
     "STAGES",
@@ -38,9 +38,17 @@ def _cm(t):
     """Column-major view parameters (ptr, ld) of a 2-D (rows, cols) tensor."""
     if t.dim() != 2:
         raise ValueError("expected a 2-D tensor")
-    if t.stride(0) == 1 and t.stride(1) >= max(1, t.shape[0]):
-        return t.data_ptr(), t.stride(1)
-    raise ValueError("matrix must be column-major: use evd4sc2025_amd.colmajor(...)")
+    rows, cols = t.shape
+    s0, s1 = t.stride()
+    if rows <= 1:
+        return t.data_ptr(), (s1 if cols > 1 else 1)
+    if s0 != 1:
+        raise ValueError("matrix must be column-major: use evd4sc2025_amd.colmajor(...)")
+    if cols <= 1:
+        return t.data_ptr(), rows
+    if s1 < rows:
+        raise ValueError("column-major leading dimension smaller than the row count")
+    return t.data_ptr(), s1
 
 
 def colmajor(rows: int, cols: int, dtype=None, device="cuda", fill=None):
@@ -74,6 +82,9 @@ class Context:
         if rc != 0:
             raise EvdError("evd_ctx_create", rc, "context creation failed")
         self.h = h
+        # bind to exactly torch's stream, including the null stream (handle 0),
+        # so library work is ordered with torch's copies and kernels
+        self.call("evd_ctx_set_stream", C.c_void_p(stream.cuda_stream))
         self.device = device
 
     def close(self):
@@ -142,6 +153,18 @@ def panelQR(ctx: Context, A, W, R):
     pw, ldw = _cm(W)
     pr, ldr = _cm(R)
     ctx.call("evd_panel_qr_f32", m, b, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(pr), ldr)
+
+
+def hou_tsqr_panel(ctx: Context, A, R):
+    """api/inc/TallShinnyQR.h:9 - A (m x n) <- explicit thin Q of the panel, R (n x n)."""
+    m, n = A.shape
+    W = colmajor(m, n)
+    work = colmajor(n, n)
+    pa, lda = _cm(A)
+    pw, ldw = _cm(W)
+    pr, ldr = _cm(R)
+    ctx.call("evd_panel_qr_f32", m, n, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(pr), ldr)
+    ctx.call("evd_orgqr_f32", m, n, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(work.data_ptr()))
 
 
 def tc_ozimmu_syr2k(ctx: Context, alpha: float, A, B, beta: float, C_):

@@ -33,6 +33,7 @@ SIGNATURES = {
     "evd_ctx_stage_ms": (_i, [_vp, C.POINTER(_f), _i]),
     "evd_ctx_synchronize": (_i, [_vp]),
     "evd_panel_qr_f32": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
+    "evd_orgqr_f32": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp]),
     "evd_syr2k_f32": (_i, [_vp, _l, _l, _f, _vp, _l, _vp, _l, _f, _vp, _l]),
     "evd_sy2sb_f32": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
     "evd_band_pack_f32": (_i, [_vp, _l, _l, _vp, _l, _vp, _l]),

@@ -98,6 +98,13 @@ struct TileLoader {
   }
 };
 
+// Index (0..15) of the MFMA D-matrix row held in accumulator element r of a
+// lane: v_mfma_f32_16x16x4f32 keeps rows 4*(lane/16)+r, v_mfma_f64_16x16x4f64
+// keeps rows lane/16 + 4*r.  With the (B, A) operand order used here the D row
+// is the C column inside the 16-wide tile.
+__device__ inline int acc_row(float, int lane, int r) { return 4 * (lane >> 4) + r; }
+__device__ inline int acc_row(double, int lane, int r) { return (lane >> 4) + 4 * r; }
+
 __device__ inline f32x4 mfma16(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(256) void dgemm_kernel(int M, int N, int K, double 
     for (int ni = 0; ni < NI; ni++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        const int col = n0 + wn * (BN / WN) + ni * 16 + acc_row(0.0, lane, r);
         if (col >= N) continue;
         long o = row + (long)col * ldc;
         C[o] = (beta == 0.0) ? alpha * acc[mi][ni][r] : alpha * acc[mi][ni][r] + beta * C[o];
@@ -433,7 +440,7 @@ __global__ __launch_bounds__(256) void dgemm_grouped_kernel(const DGemmProblem *
     for (int ni = 0; ni < NI; ni++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        const int col = n0 + wn * (BN / WN) + ni * 16 + acc_row(0.0, lane, r);
         if (col >= p.n) continue;
         const int oc = p.colmap ? p.colmap[col] : col;
         p.C[row + (long)oc * p.ldc] = acc[mi][ni][r];

@@ -239,3 +239,19 @@ extern "C" int evd_sbr_back_f32(evd_ctx *ctx, long n, long b, long ncols, float 
   EVD_TRY(wy_merge_doubling(ctx, n, b, bb, n, W, ldw, Y, ldy));
   return sbr_apply(ctx, n, b, bb, n, ncols, Z, ldz, W, ldw, Y, ldy);
 }
+
+extern "C" int evd_orgqr_f32(evd_ctx *ctx, long m, long n, float *Q, long ldq, const float *W, long ldw, float *work) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, m >= 0, 2);
+  EVD_ARG(ctx, n >= 0 && n <= m, 3);
+  EVD_ARG(ctx, ldq >= std::max<long>(1, m), 5);
+  EVD_ARG(ctx, ldw >= std::max<long>(1, m), 7);
+  EVD_ARG(ctx, work != nullptr || n == 0, 8);
+  if (n == 0) return 0;
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)64 << 20));
+  EVD_TRY(copy_f32(ctx, Q, ldq, work, n, n, n));
+  EVD_TRY(sgemm(ctx, 'N', 'T', m, n, n, -1.f, W, ldw, work, n, 0.f, Q, ldq));
+  add_identity_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, Q, ldq);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}

@@ -54,13 +54,17 @@ struct DCScratch {
   double *Qc, *Uc;  // n x n, ld n
 };
 
-__global__ void dc_tear_kernel(const MergeDesc *md, int nm, double *D) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nm) return;
-  MergeDesc m = md[t];
-  int k = m.start + m.n1;
-  D[k - 1] -= m.rho;
-  D[k] -= m.rho;
+// Tearing of every split at once, one thread per diagonal entry (an entry
+// between two splits is adjusted by both): split[k] != 0 marks a split between
+// rows k-1 and k.
+__global__ void dc_tear_kernel(long n, const int *__restrict__ split, const double *__restrict__ E,
+                               double *__restrict__ D) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double d = D[i];
+    if (i > 0 && split[i]) d -= fabs(E[i - 1]);
+    if (i + 1 < n && split[i + 1]) d -= fabs(E[i]);
+    D[i] = d;
+  }
 }
 
 __global__ void dc_sort_kernel(const MergeDesc *__restrict__ md, const double *__restrict__ D,
@@ -558,8 +562,13 @@ int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
   unsigned *err = ctx->ctrl + kCtrlError;
 
   // tearing of every split, then Q = I
-  dc_tear_kernel<<<(nmerge + 255) / 256, 256, 0, ctx->stream>>>(dmd, nmerge, D);
+  std::vector<int> hsplit(n, 0);
+  for (const MergeDesc &m : all) hsplit[m.start + m.n1] = 1;
+  int *dsplit = sc.colmap;  // scratch until the first merge writes column maps
+  EVD_HIP(ctx, hipMemcpyAsync(dsplit, hsplit.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  dc_tear_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, dsplit, E, D);
   EVD_LAUNCHED(ctx);
+  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));  // hsplit is pageable host memory
   EVD_TRY(set_identity_f64(ctx, Z, n, ldz));
 
   std::vector<MergeOut> hmo(nmerge);

@@ -85,6 +85,12 @@ int evd_ctx_synchronize(evd_ctx *ctx);
  * and hou_tsqr_panel (api/inc/TallShinnyQR.h:9-134). */
 int evd_panel_qr_f32(evd_ctx *ctx, long m, long nb, float *A, long lda, float *W, long ldw, float *R, long ldr);
 
+/* Explicit thin Q (m x n) of a panel factorisation: Q = [I_n; 0] - W Y_1^T,
+ * where Q holds Y (unit lower, as left by evd_panel_qr_f32) on entry and Y_1
+ * is its leading n x n block; work holds n*n floats.  The explicit-Q output of
+ * hou_tsqr_panel (api/inc/TallShinnyQR.h:9-134). */
+int evd_orgqr_f32(evd_ctx *ctx, long m, long n, float *Q, long ldq, const float *W, long ldw, float *work);
+
 /* Symmetric rank-2k update of the lower triangle, mirrored to the upper:
  * C = alpha (A B^T + B A^T) + beta C, A and B n x k.
  * Replaces tc_ozimmu_syr2k<T> (api/TCblas/tc_ozimmu_syr2k.cu:209-341,

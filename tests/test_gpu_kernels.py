@@ -217,3 +217,19 @@ def test_wy_from_slide_and_q_from_wy(ctx):
     dQ = E.colmajor(M, M)
     E.computerQFromWY(ctx, dQ, dW, dY)
     assert np.abs(to_host(dQ) - want).max() < 1e-4
+
+
+@pytest.mark.parametrize("m,b", [(300, 32), (2000, 64)])
+def test_hou_tsqr_panel_explicit_q(ctx, m, b):
+    import evd4sc2025_amd as E
+
+    P = O.gen_symmetric(m, 70 + m)[:, :b]
+    dA = to_dev(P)
+    dR = E.colmajor(b, b, fill=0.0)
+    E.hou_tsqr_panel(ctx, dA, dR)
+    Q = to_host(dA).astype(np.float64)
+    R = np.triu(to_host(dR).astype(np.float64))
+    assert O.orthogonality(Q) < 1e-6
+    assert np.abs(Q @ R - P).max() < 1e-5 * np.sqrt(m)
+    _, _, R0, _ = O.panel_qr(P.astype(np.float64))
+    np.testing.assert_allclose(R, R0, atol=1e-4)
