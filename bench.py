@@ -88,9 +88,11 @@ def main():
         Aw.copy_(A0)
         return E.syevd(ctx, Aw, b=b, nb=nb, jobz="V", overwrite=True)
 
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
+        t = time.perf_counter()
         step()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {i}: {time.perf_counter() - t:.3f} s", file=sys.stderr, flush=True)
     stage_tot = {k: 0.0 for k in E.STAGES}
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -125,6 +127,8 @@ def main():
                "trace_rel_err": float(abs(r.w.sum().item() - torch.trace(A0.double()).item()) /
                                       max(1.0, torch.linalg.matrix_norm(A0.double()).item()))}
 
+    print(f"[bench] timed {args.steps} steps: {dt:.3f} s/EVD, stages {stage_ms}, accuracy {acc}", file=sys.stderr,
+          flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     line = {
         "metric": "EVD s & TFLOP/s at n=16384 FP32 b=32; mixed-size solves/s + p99 latency",

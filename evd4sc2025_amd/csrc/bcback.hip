@@ -23,36 +23,51 @@
 
 namespace evd {
 
-// Reflector TT of block (S0, step) applied to the register window; the
-// recursion keeps every window index a compile-time constant.
+// Fast path: every reflector of the block exists and lies inside the matrix,
+// so the entries are used straight from SGPRs as FMA operands.
+template <int NB, int BS, int W, bool REV, int TT>
+__device__ __attribute__((always_inline)) inline void apply_block_full(float (&z)[W], const float *__restrict__ U,
+                                                                       long ldu, int S0, int wb) {
+  if constexpr (TT < BS) {
+    constexpr int t = REV ? (BS - 1 - TT) : TT;
+    const float *up = U + (wb + t) + (long)(S0 + t) * ldu;
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f;
+#pragma unroll
+    for (int i = 0; i < NB; i += 4) {
+      d0 += up[i] * z[t + i];
+      d1 += up[i + 1] * z[t + i + 1];
+      d2 += up[i + 2] * z[t + i + 2];
+      d3 += up[i + 3] * z[t + i + 3];
+    }
+    const float dot = (d0 + d1) + (d2 + d3);
+#pragma unroll
+    for (int i = 0; i < NB; i++) z[t + i] -= dot * up[i];
+    apply_block_full<NB, BS, W, REV, TT + 1>(z, U, ldu, S0, wb);
+  }
+}
+
+// General path (blocks at the matrix edge): a reflector is skipped when its
+// sweep or its start row does not exist.  Rows of U at and below n must be
+// zero (the pipeline's U is zero-filled; the drop-in entry pads a copy), so no
+// per-entry masking is needed.
 template <int NB, int BS, int W, bool REV, int TT>
 __device__ __attribute__((always_inline)) inline void apply_block(float (&z)[W], const float *__restrict__ U,
                                                                   long ldu, int S0, int wb, int n, int nsweeps) {
   if constexpr (TT < BS) {
     constexpr int t = REV ? (BS - 1 - TT) : TT;
-    const int s = S0 + t;
-    const int p = wb + t;
-    if (s < nsweeps && p < n - 1) {
-      const float *up = U + p + (long)s * ldu;
-      float uu[NB];
-#pragma unroll
-      for (int i = 0; i < NB; i++) uu[i] = up[i];
-      if (p + NB > n) {
-#pragma unroll
-        for (int i = 0; i < NB; i++)
-          if (p + i >= n) uu[i] = 0.f;
-      }
+    if (S0 + t < nsweeps && wb + t < n - 1) {
+      const float *up = U + (wb + t) + (long)(S0 + t) * ldu;
       float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f;
 #pragma unroll
       for (int i = 0; i < NB; i += 4) {
-        d0 += uu[i] * z[t + i];
-        d1 += uu[i + 1] * z[t + i + 1];
-        d2 += uu[i + 2] * z[t + i + 2];
-        d3 += uu[i + 3] * z[t + i + 3];
+        d0 += up[i] * z[t + i];
+        d1 += up[i + 1] * z[t + i + 1];
+        d2 += up[i + 2] * z[t + i + 2];
+        d3 += up[i + 3] * z[t + i + 3];
       }
       const float dot = (d0 + d1) + (d2 + d3);
 #pragma unroll
-      for (int i = 0; i < NB; i++) z[t + i] -= dot * uu[i];
+      for (int i = 0; i < NB; i++) z[t + i] -= dot * up[i];
     }
     apply_block<NB, BS, W, REV, TT + 1>(z, U, ldu, S0, wb, n, nsweeps);
   }
@@ -140,7 +155,10 @@ __global__ __launch_bounds__(NW * 64) void bcback_kernel(int n, int ncols, float
           }
         }
         // ---- apply the BS reflectors of this block at step jj
-        apply_block<NB, BS, W, REV, 0>(z, U, ldu, S0, wb, n, nsweeps);
+        if (S0 + BS <= nsweeps && wb + BS + NB <= n)
+          apply_block_full<NB, BS, W, REV, 0>(z, U, ldu, S0, wb);
+        else
+          apply_block<NB, BS, W, REV, 0>(z, U, ldu, S0, wb, n, nsweeps);
         // ---- hand on the finished rows
         const bool last = (k == nact - 1);
         if (REV) {
@@ -193,11 +211,16 @@ static int launch_bcback(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, c
 
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev) {
   const int bs = ctx->tune.bc_sweeps;
+  const int nw = ctx->tune.bc_waves;
   if (b == 32) {
-    if (bs == 64) return launch_bcback<32, 64, 4>(ctx, n, ncols, Q, ldq, U, ldu, rev);
-    return launch_bcback<32, 32, 4>(ctx, n, ncols, Q, ldq, U, ldu, rev);
+    if (bs == 32) {
+      if (nw == 4) return launch_bcback<32, 32, 4>(ctx, n, ncols, Q, ldq, U, ldu, rev);
+      return launch_bcback<32, 32, 8>(ctx, n, ncols, Q, ldq, U, ldu, rev);
+    }
+    if (nw == 4) return launch_bcback<32, 64, 4>(ctx, n, ncols, Q, ldq, U, ldu, rev);
+    return launch_bcback<32, 64, 8>(ctx, n, ncols, Q, ldq, U, ldu, rev);
   }
-  if (b == 64) return launch_bcback<64, 64, 4>(ctx, n, ncols, Q, ldq, U, ldu, rev);
+  if (b == 64) return launch_bcback<64, 64, 4>(ctx, n, ncols, Q, ldq, U, ldu, rev);  // 8 waves exceed LDS
   return set_error(ctx, EVD_E_UNSUPPORTED, "BC back transformation needs b = 32 or 64 (got %ld)", b);
 }
 
@@ -214,5 +237,15 @@ extern "C" int evd_bc_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *
   EVD_ARG(ctx, ldq >= std::max<long>(1, n), 6);
   EVD_ARG(ctx, ldu >= n, 8);
   EVD_ARG(ctx, order == 0 || order == 1, 9);
-  return bc_back(ctx, n, b, ncols, Q, ldq, U, ldu, order == 1);
+  if (n <= 2 || ncols == 0) return 0;
+  // the kernel reads up to b rows past the last reflector: run on a copy of
+  // the caller's U whose rows n .. n+b-1 are zero
+  const long ldp = n + b;
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)ldp * n * sizeof(float) + 4096));
+  ArenaFrame frame(ctx);
+  float *Up = arena_alloc_n<float>(ctx, (size_t)ldp * n);
+  if (!Up) return EVD_E_NOMEM;
+  EVD_TRY(fill_f32(ctx, Up, ldp, n, ldp, 0.f));
+  EVD_TRY(copy_f32(ctx, U, ldu, Up, ldp, n, n));
+  return bc_back(ctx, n, b, ncols, Q, ldq, Up, ldp, order == 1);
 }

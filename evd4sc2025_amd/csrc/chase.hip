@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void chase_kernel(int n, int b, double *__rest
         if (s > 0) {
           const int need = p + 2 * b;
           unsigned long long t0 = clock100();
-          while (ld_acquire(&com[s - 1]) < need) {
+          while (ld_dev(&com[s - 1]) < need) {
             __builtin_amdgcn_s_sleep(1);
             if (clock100() - t0 > kWatchdogTicks) {
               atomicOr(err, (unsigned)EVD_E_TIMEOUT);
@@ -62,13 +62,12 @@ __global__ __launch_bounds__(256) void chase_kernel(int n, int b, double *__rest
       }
       __syncthreads();
       if (!s_ok) {
-        if (tid == 0) st_release(&com[s], INT_MAX);
+        if (tid == 0) st_dev(&com[s], INT_MAX);
         return;
       }
-      __threadfence();
       if (first) {
         // column s is final once sweep s-1 has passed row s+1+2b
-        for (int i = tid; i < len; i += 256) B1[i * LD + 0] = band[(p + i - s) + (long)s * ldb];
+        for (int i = tid; i < len; i += 256) B1[i * LD + 0] = ld_dev(&band[(p + i - s) + (long)s * ldb]);
         first = false;
       }
       const int len2 = max(0, min(b, n - p - len));
@@ -76,14 +75,14 @@ __global__ __launch_bounds__(256) void chase_kernel(int n, int b, double *__rest
       for (int idx = tid; idx < len * len; idx += 256) {
         int i = idx % len, kk = idx / len;
         if (i >= kk) {
-          double v = band[(i - kk) + (long)(p + kk) * ldb];
+          double v = ld_dev(&band[(i - kk) + (long)(p + kk) * ldb]);
           S[i * LD + kk] = v;
           S[kk * LD + i] = v;
         }
       }
       for (int idx = tid; idx < len2 * len; idx += 256) {
         int i = idx % len2, kk = idx / len2;
-        Nb[i * LD + kk] = band[(len + i - kk) + (long)(p + kk) * ldb];
+        Nb[i * LD + kk] = ld_dev(&band[(len + i - kk) + (long)(p + kk) * ldb]);
       }
       __syncthreads();
       // ---- reflector from B1[:, 0]
@@ -144,22 +143,22 @@ __global__ __launch_bounds__(256) void chase_kernel(int n, int b, double *__rest
       // ---- write back B1 and S (and N at the end of the sweep)
       for (int idx = tid; idx < len * colB1; idx += 256) {
         int i = idx % len, c = idx / len;
-        band[(p + i - (cB1 + c)) + (long)(cB1 + c) * ldb] = B1[i * LD + c];
+        st_dev(&band[(p + i - (cB1 + c)) + (long)(cB1 + c) * ldb], B1[i * LD + c]);
       }
       for (int idx = tid; idx < len * len; idx += 256) {
         int i = idx % len, kk = idx / len;
-        if (i >= kk) band[(i - kk) + (long)(p + kk) * ldb] = S[i * LD + kk];
+        if (i >= kk) st_dev(&band[(i - kk) + (long)(p + kk) * ldb], S[i * LD + kk]);
       }
       const bool last = (len2 <= 1);
       if (last) {
         for (int idx = tid; idx < len2 * len; idx += 256) {
           int i = idx % len2, kk = idx / len2;
-          band[(len + i - kk) + (long)(p + kk) * ldb] = Nb[i * LD + kk];
+          st_dev(&band[(len + i - kk) + (long)(p + kk) * ldb], Nb[i * LD + kk]);
         }
       }
-      __threadfence();
+      wait_mem();
       __syncthreads();
-      if (tid == 0) st_release(&com[s], last ? INT_MAX : p + len);
+      if (tid == 0) st_dev(&com[s], last ? INT_MAX : p + len);
       if (last) break;
       double *t = B1;
       B1 = Nb;

@@ -44,6 +44,8 @@ struct evd_ctx {
   unsigned *ctrl = nullptr;  // kCtrlWords words, zeroed at creation
   size_t ctrl_words = 0;
 
+  unsigned epoch = 0;        // launch counter for tag-based exchanges
+  unsigned *tags = nullptr;  // kTagWords persistent exchange tags, zeroed at creation
   evd::Tuning tune;
   int precision = EVD_PREC_FP32;
 
@@ -61,6 +63,7 @@ enum : int {
   kCtrlError = 0,      // device-side error word (EVD_E_*), sticky until read
   kCtrlBarrier = 16,   // grid barrier: [count, generation] pairs, 64 slots
   kCtrlWordsTotal = 16 + 2 * 64,
+  kTagWords = 4096,
 };
 
 int set_error(evd_ctx *ctx, int code, const char *fmt, ...);
@@ -174,28 +177,47 @@ __device__ inline void st_release(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Device-coherent (agent scope) relaxed accesses.  Data that one workgroup
+// hands to another inside a persistent kernel goes through these instead of
+// plain loads/stores + release/acquire fences: on a multi-XCD part an
+// agent-scope fence writes back / invalidates the whole L2, while these
+// accesses are coherent by themselves.  Ordering of such stores before a flag
+// store is the writer's s_waitcnt (all stores acknowledged) + barrier.
+template <typename T>
+__device__ inline T ld_dev(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline void st_dev(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
+
 // 100 MHz constant clock -> bounded waits (watchdog) in persistent kernels
 __device__ inline unsigned long long clock100() { return wall_clock64(); }
 constexpr unsigned long long kWatchdogTicks = 100ull * 1000ull * 1000ull * 20ull;  // 20 s
 
-// Sense-free generation barrier over `nblocks` co-resident workgroups.
-// words[0] = arrival count (returns to 0), words[1] = generation.  Every
-// thread of every block must call it.  Returns false if the watchdog expired
-// (the caller must then leave the kernel).
+// Generation barrier over `nblocks` co-resident workgroups for kernels that
+// exchange data only through ld_dev/st_dev.  words[0] = arrival count
+// (returns to 0), words[1] = generation.  Every thread of every block must
+// call it.  Returns false if the watchdog expired (the caller must then leave
+// the kernel; the other blocks time out the same way).
 __device__ inline bool grid_barrier(unsigned *words, unsigned nblocks, unsigned *err) {
-  __threadfence();
+  wait_mem();
   __syncthreads();
   __shared__ int s_ok;
   if (threadIdx.x == 0) {
     s_ok = 1;
-    unsigned gen = __hip_atomic_load(&words[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned arrived = __hip_atomic_fetch_add(&words[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    const unsigned gen = ld_dev(&words[1]);
+    wait_mem();
+    const unsigned arrived = __hip_atomic_fetch_add(&words[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
     if (arrived == nblocks) {
-      __hip_atomic_store(&words[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&words[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      st_dev(&words[0], 0u);
+      wait_mem();
+      __hip_atomic_fetch_add(&words[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      unsigned long long t0 = clock100();
-      while (__hip_atomic_load(&words[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      const unsigned long long t0 = clock100();
+      while (ld_dev(&words[1]) == gen) {
         __builtin_amdgcn_s_sleep(1);
         if (clock100() - t0 > kWatchdogTicks) {
           atomicOr(err, (unsigned)EVD_E_TIMEOUT);
@@ -206,7 +228,6 @@ __device__ inline bool grid_barrier(unsigned *words, unsigned nblocks, unsigned 
     }
   }
   __syncthreads();
-  __threadfence();
   return s_ok != 0;
 }
 

@@ -170,6 +170,8 @@ int evd_ctx_create(int device, void *stream, evd_ctx **out) {
   ctx->ctrl_words = kCtrlWordsTotal;
   if (hipMalloc(&ctx->ctrl, ctx->ctrl_words * sizeof(unsigned)) != hipSuccess ||
       hipMemset(ctx->ctrl, 0, ctx->ctrl_words * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&ctx->tags, kTagWords * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(ctx->tags, 0, kTagWords * sizeof(unsigned)) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     delete ctx;
     return EVD_E_HIP;
@@ -188,6 +190,7 @@ int evd_ctx_destroy(evd_ctx *ctx) {
   hipStreamSynchronize(ctx->stream);
   if (ctx->arena) hipFree(ctx->arena);
   if (ctx->ctrl) hipFree(ctx->ctrl);
+  if (ctx->tags) hipFree(ctx->tags);
   for (auto &e : ctx->ev)
     if (e) hipEventDestroy(e);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);

@@ -58,12 +58,15 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
                                                        long ldw, float *__restrict__ R, long ldr,
                                                        float *__restrict__ Aout, long ldo, double *__restrict__ part,
                                                        double *__restrict__ pivrow, double *__restrict__ tau_out,
-                                                       unsigned *__restrict__ bar, unsigned *__restrict__ err) {
+                                                       unsigned *__restrict__ tags, unsigned epoch,
+                                                       unsigned *__restrict__ err) {
   const int G = gridDim.x;
   const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = min(m, nc);
   __shared__ double red[4][NB];
   __shared__ double tot[NB];
+  __shared__ double gpart[32 * NB];  // partial records of up to 32 groups per pass
+  __shared__ int s_ok;
   __shared__ double piv[NB];
   __shared__ double wv[NB];
   __shared__ double sc[4];
@@ -84,10 +87,48 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
     (&gram[0][0])[i] = 0.0;
     (&T[0][0])[i] = 0.0;
   }
+  if (G > 1) {
+    // sentinels of steps 0 and 1 in this group's records, then a one-time
+    // start barrier so nobody polls a slot before its owner marked it
+    const long rec = 2 * NB;
+    for (int st = 0; st < 2; st++) {
+      const unsigned long long snt = 0x7FF8000000000000ull | ((unsigned long long)(epoch & 0xFFFFFu) << 24) |
+                                     (unsigned long long)st;
+      double *r0 = part + ((long)st * G + g) * rec;
+      for (int c = tid; c < (int)rec; c += 256) st_dev(&r0[c], __longlong_as_double((long long)snt));
+    }
+    wait_mem();
+    __syncthreads();
+    if (tid == 0) {
+      s_ok = 1;
+      st_dev(&tags[g], epoch);
+    }
+    __syncthreads();
+    for (int gg = tid; gg < G; gg += 256) {
+      const unsigned long long t0 = clock100();
+      while (ld_dev(&tags[gg]) != epoch) {
+        __builtin_amdgcn_s_sleep(1);
+        if (clock100() - t0 > kWatchdogTicks) {
+          atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+          s_ok = 0;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (!s_ok) return;
+  }
 
   for (int j = 0; j <= k; j++) {
     // ---- partials: gram(i, j-1) for i < j-1; sigma_j; p_c (c > j), in chunks
     // of 16 columns to bound register pressure
+    double vprevq[RPT], xq[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const long r = rowv[q];
+      vprevq[q] = (r < m && j >= 1) ? ((r > j - 1) ? sel<NB>(a[q], j - 1) : (r == j - 1 ? 1.0 : 0.0)) : 0.0;
+      xq[q] = (r < m && j < k && r > j) ? sel<NB>(a[q], j) : 0.0;
+    }
 #pragma unroll
     for (int cb = 0; cb < NB; cb += 16) {
       double pv[16];
@@ -97,8 +138,8 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
       for (int q = 0; q < RPT; q++) {
         const long r = rowv[q];
         if (r < m) {
-          const double vprev = (j >= 1) ? ((r > j - 1) ? sel<NB>(a[q], j - 1) : (r == j - 1 ? 1.0 : 0.0)) : 0.0;
-          const double x = (j < k && r > j) ? sel<NB>(a[q], j) : 0.0;
+          const double vprev = vprevq[q];
+          const double x = xq[q];
 #pragma unroll
           for (int cc = 0; cc < 16; cc++) {
             const int c = cb + cc;
@@ -117,21 +158,83 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
       if ((lane & 3) == 0) red[wave][cb + (lane >> 2)] = mine;
     }
     __syncthreads();
-    if (tid < NB) {
-      double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-      part[((long)(j & 1) * G + g) * NB + tid] = s;
-    }
-    // pivot row j is held by workgroup 0, thread j, q 0
-    if (g == 0 && tid == j && j < k) {
+    // ---- exchange.  One workgroup: straight from LDS.  Several: every group
+    // stores its record (partials, and the pivot row from group 0) into ring
+    // slot j%4 with device-coherent stores; readers poll the values themselves
+    // until none equals the slot's sentinel (a NaN whose payload encodes the
+    // launch epoch and the step).  The sentinel of step j+2 is written at step
+    // j, into the slot that held step j-2, which every reader has consumed by
+    // then; the vmcnt wait of this step's polling orders it before the values
+    // of step j+1.
+    if (G == 1) {
+      if (tid < NB) tot[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+      if (tid == j && j < k) {
 #pragma unroll
-      for (int c = 0; c < NB; c++) pivrow[(j & 1) * NB + c] = a[0][c];
-    }
-    if (!grid_barrier(bar, (unsigned)G, err)) return;
-    if (tid < NB) {
-      double s = 0.0;
-      for (int gg = 0; gg < G; gg++) s += part[((long)(j & 1) * G + gg) * NB + tid];
-      tot[tid] = s;
-      piv[tid] = pivrow[(j & 1) * NB + tid];
+        for (int c = 0; c < NB; c++) piv[c] = a[0][c];
+      }
+      __syncthreads();
+    } else {
+      const int slot = j & 3;
+      const long rec = 2 * NB;
+      double *myrec = part + ((long)slot * G + g) * rec;
+      if (tid < NB) st_dev(&myrec[tid], red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
+      if (g == 0 && tid == j && j < k) {
+#pragma unroll
+        for (int c = 0; c < NB; c++) st_dev(&myrec[NB + c], a[0][c]);
+      }
+      // sentinel for step j+2 into its slot (this group's record only)
+      {
+        const unsigned long long snt = 0x7FF8000000000000ull | ((unsigned long long)(epoch & 0xFFFFFu) << 24) |
+                                       (unsigned long long)((j + 2) & 0xFFFFFF);
+        double *nrec = part + ((long)((j + 2) & 3) * G + g) * rec;
+        for (int c = tid; c < (int)rec; c += 256) st_dev(&nrec[c], __longlong_as_double((long long)snt));
+      }
+      const unsigned long long want = 0x7FF8000000000000ull | ((unsigned long long)(epoch & 0xFFFFFu) << 24) |
+                                      (unsigned long long)(j & 0xFFFFFF);
+      if (tid == 0) s_ok = 1;
+      if (tid < NB) tot[tid] = 0.0;
+      for (int g0 = 0; g0 < G; g0 += 32) {
+        const int gn = min(32, G - g0);
+        __syncthreads();
+        for (int idx = tid; idx < gn * NB; idx += 256) {
+          const double *src = part + ((long)slot * G + g0 + idx / NB) * rec + (idx % NB);
+          double v = ld_dev(src);
+          const unsigned long long t0 = clock100();
+          while ((unsigned long long)__double_as_longlong(v) == want) {
+            __builtin_amdgcn_s_sleep(1);
+            v = ld_dev(src);
+            if (clock100() - t0 > kWatchdogTicks) {
+              atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+              s_ok = 0;
+              break;
+            }
+          }
+          gpart[idx] = v;
+        }
+        __syncthreads();
+        if (tid < NB) {
+          double sacc = tot[tid];
+          for (int gg = 0; gg < gn; gg++) sacc += gpart[gg * NB + tid];
+          tot[tid] = sacc;
+        }
+      }
+      if (j < k && tid < NB) {
+        const double *src = part + ((long)slot * G + 0) * rec + NB + tid;
+        double v = ld_dev(src);
+        const unsigned long long t0 = clock100();
+        while ((unsigned long long)__double_as_longlong(v) == want) {
+          __builtin_amdgcn_s_sleep(1);
+          v = ld_dev(src);
+          if (clock100() - t0 > kWatchdogTicks) {
+            atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+            s_ok = 0;
+            break;
+          }
+        }
+        piv[tid] = v;
+      }
+      __syncthreads();
+      if (!s_ok) return;
     }
     __syncthreads();
     if (tid < NB && tid < j - 1) gram[tid][j - 1] = tot[tid];
@@ -234,13 +337,17 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
   EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<NB, RPT>, 256, 0, &maxres));
   if (G > maxres) return set_error(ctx, EVD_E_UNSUPPORTED, "panel of %ld rows needs %d co-resident groups (max %d)", m, G, maxres);
   ArenaFrame frame(ctx);
-  double *part = arena_alloc_n<double>(ctx, (size_t)2 * G * NB);
+  double *part = arena_alloc_n<double>(ctx, (size_t)4 * G * 2 * NB);
   double *piv = arena_alloc_n<double>(ctx, (size_t)2 * NB);
   if (!part || !piv) return EVD_E_NOMEM;
-  unsigned *bar = ctx->ctrl + kCtrlBarrier;
+  if (G > kTagWords) return set_error(ctx, EVD_E_UNSUPPORTED, "panel needs %d exchange tags", G);
+  unsigned *tags = ctx->tags;
+  // tags carry a launch epoch, so stale tags of earlier launches never match
+  unsigned epoch = (++ctx->epoch) & 0x1FFFFFFu;
+  if (epoch == 0) epoch = (++ctx->epoch) & 0x1FFFFFFu;
   unsigned *err = ctx->ctrl + kCtrlError;
   int mi = (int)m, ni = (int)nc;
-  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &piv, &tau_out, &bar, &err};
+  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &piv, &tau_out, &tags, &epoch, &err};
   EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0,
                                           ctx->stream));
   return 0;
