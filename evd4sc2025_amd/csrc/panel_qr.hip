@@ -5,97 +5,108 @@
 // Design (MI355X): one cooperative launch of G workgroups; workgroup g keeps
 // rows [g*256*RPT, (g+1)*256*RPT) of the m x nb panel in registers as FP64 for
 // the whole factorisation (the panel is read once and written once).  Column
-// j needs sigma = ||x(j+1:)||^2 and p_c = x(j+1:)^T a_c(j+1:) for the trailing
-// columns; both are sums over rows below the pivot and are reduced in ONE
-// grid-wide reduction per column, together with the Gram entries v_i^T v_{j-1}
-// of the previous reflector.  After the barrier every workgroup forms the
-// same LAPACK dlarfg reflector (beta = -sign(alpha) ||x||, tau = (beta -
-// alpha)/beta, v = x / (alpha - beta)) and applies it to its rows.  The Gram
-// matrix V^T V gives T by the dlarft forward/columnwise recurrence; W = V T is
-// formed row-locally.  Outputs: Y = V (unit lower, zeros above), W = V T, R,
-// and optionally [R; 0] into the panel storage.
+// j needs sigma = ||x||^2 and p_c = x^T a_c for the trailing columns, x = the
+// pivot column below the diagonal; both are sums over rows below the pivot
+// and are reduced in ONE exchange per column, which also yields x^T v_i for
+// the finished reflectors and with the pivot row the Gram column v_i^T v_j.
+// After the exchange every workgroup forms the same LAPACK dlarfg reflector
+// (beta = -sign(alpha) ||x||, tau = (beta - alpha)/beta, v = x / (alpha -
+// beta)) and applies it to its rows.  The Gram matrix V^T V gives T by the
+// dlarft forward/columnwise recurrence; W = V T is formed row-locally.
+// Outputs: Y = V (unit lower, zeros above), W = V T, R, and optionally [R; 0]
+// into the panel storage.
 #include "common.hpp"
 
 namespace evd {
 
-// Reduce NV (power of two, <= 64) per-lane values across the wave.  On exit
-// lane l holds the total of value index l / (64 / NV) in v[0].
-template <int NV>
-__device__ inline double wave_reduce_scatter(double (&v)[NV], int lane) {
-#pragma unroll
-  for (int L = 0; (NV >> L) > 1; L++) {
-    constexpr int dummy = 0;
-    (void)dummy;
-    const int o = 32 >> L;
-    const int half = (NV >> L) >> 1;
+// One butterfly level of a wave reduce-scatter over N live values: lanes with
+// bit o set keep the upper half, the others the lower half, and each adds the
+// half its partner kept.  Template recursion keeps every index static.
+template <int NV, int N>
+__device__ inline void reduce_scatter_level(double (&v)[NV], int lane) {
+  if constexpr (N > 1) {
+    constexpr int half = N / 2;
+    constexpr int o = 32 * N / NV;
     const bool up = (lane & o) != 0;
 #pragma unroll
-    for (int i = 0; i < NV / 2; i++) {
-      if (i < half) {
-        double send = up ? v[i] : v[i + half];
-        double keep = up ? v[i + half] : v[i];
-        v[i] = keep + __shfl_xor(send, o, 64);
-      }
+    for (int i = 0; i < half; i++) {
+      const double send = up ? v[i] : v[i + half];
+      const double keep = up ? v[i + half] : v[i];
+      v[i] = keep + __shfl_xor(send, o, 64);
     }
+    reduce_scatter_level<NV, half>(v, lane);
   }
-  // lanes that share an index: the low log2(64/NV) lane bits
-#pragma unroll
-  for (int o = (64 / NV) >> 1; o > 0; o >>= 1) v[0] += __shfl_xor(v[0], o, 64);
-  return v[0];
 }
 
-template <int NB>
-__device__ inline double sel(const double (&row)[NB], int j) {
-  double x = 0.0;
+// Reduce NV (power of two, <= 64) per-lane values across the wave.  On exit
+// lane l holds the total of value index l / (64 / NV).
+template <int NV>
+__device__ inline double wave_reduce_scatter(double (&v)[NV], int lane) {
+  reduce_scatter_level<NV, NV>(v, lane);
+  double r = v[0];
 #pragma unroll
-  for (int c = 0; c < NB; c++) x = (c == j) ? row[c] : x;
-  return x;
+  for (int o = (64 / NV) >> 1; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+  return r;
 }
 
+constexpr unsigned long long panel_sentinel(unsigned epoch, int step) {
+  return 0x7FF8000000000000ull | ((unsigned long long)(epoch & 0xFFFFFu) << 24) |
+         (unsigned long long)((unsigned)step & 0xFFFFFFu);
+}
+
+// Register layout: every thread holds RPT rows of the panel, a[q][s], s a
+// rotating slot index.  At the start of step j, column j + s sits in slot s
+// for s < NB - j (the active columns; slot 0 is the pivot column) and
+// finished column i sits in slot NB - j + i.  Each step writes the finished
+// pivot column into the last slot while shifting the others down by one, so
+// every register index is a compile-time constant and the step body is a
+// short branch-free loop (a runtime column index would force select chains
+// or private memory, and a fully unrolled column loop a code footprint far
+// beyond the instruction cache).
 template <int NB, int RPT>
 __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const float *__restrict__ Ain, long lda,
                                                        float *__restrict__ Y, long ldy, float *__restrict__ W,
                                                        long ldw, float *__restrict__ R, long ldr,
                                                        float *__restrict__ Aout, long ldo, double *__restrict__ part,
-                                                       double *__restrict__ pivrow, double *__restrict__ tau_out,
-                                                       unsigned *__restrict__ tags, unsigned epoch,
-                                                       unsigned *__restrict__ err) {
+                                                       double *__restrict__ tau_out, unsigned *__restrict__ tags,
+                                                       unsigned epoch, unsigned *__restrict__ err) {
+  constexpr int CH = NB < 32 ? NB : 32;  // slots per in-wave reduction
+  constexpr int PASSG = 32;              // group records gathered per pass
+  static_assert(NB <= 256, "the pivot rows must live in group 0");
   const int G = gridDim.x;
   const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = min(m, nc);
+  const long rec = 2 * NB;  // exchange record: NB partial sums, NB pivot-row entries
   __shared__ double red[4][NB];
   __shared__ double tot[NB];
-  __shared__ double gpart[32 * NB];  // partial records of up to 32 groups per pass
-  __shared__ int s_ok;
   __shared__ double piv[NB];
   __shared__ double wv[NB];
-  __shared__ double sc[4];
-  __shared__ double gram[NB][NB + 1];
+  __shared__ double sc[2];
   __shared__ double taus[NB];
-  __shared__ double T[NB][NB + 1];
+  __shared__ double gram[NB][NB + 1];  // gram[i][j] = v_i^T v_j, i < j
+  __shared__ double Tt[NB][NB + 1];    // Tt[c][l] = T[l][c]
+  __shared__ double gpart[(PASSG + 1) * NB];
+  __shared__ int s_ok;
 
   double a[RPT][NB];
   long rowv[RPT];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
-    long r = (long)g * 256 * RPT + q * 256 + tid;
+    const long r = (long)g * 256 * RPT + q * 256 + tid;
     rowv[q] = r;
 #pragma unroll
     for (int c = 0; c < NB; c++) a[q][c] = (r < m && c < nc) ? (double)Ain[r + (long)c * lda] : 0.0;
   }
   for (int i = tid; i < NB * (NB + 1); i += 256) {
     (&gram[0][0])[i] = 0.0;
-    (&T[0][0])[i] = 0.0;
+    (&Tt[0][0])[i] = 0.0;
   }
   if (G > 1) {
     // sentinels of steps 0 and 1 in this group's records, then a one-time
     // start barrier so nobody polls a slot before its owner marked it
-    const long rec = 2 * NB;
     for (int st = 0; st < 2; st++) {
-      const unsigned long long snt = 0x7FF8000000000000ull | ((unsigned long long)(epoch & 0xFFFFFu) << 24) |
-                                     (unsigned long long)st;
       double *r0 = part + ((long)st * G + g) * rec;
-      for (int c = tid; c < (int)rec; c += 256) st_dev(&r0[c], __longlong_as_double((long long)snt));
+      for (int c = tid; c < (int)rec; c += 256) st_dev(&r0[c], __longlong_as_double((long long)panel_sentinel(epoch, st)));
     }
     wait_mem();
     __syncthreads();
@@ -119,211 +130,213 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
     if (!s_ok) return;
   }
 
-  for (int j = 0; j <= k; j++) {
-    // ---- partials: gram(i, j-1) for i < j-1; sigma_j; p_c (c > j), in chunks
-    // of 16 columns to bound register pressure
-    double vprevq[RPT], xq[RPT];
+  for (int j = 0; j < k; j++) {
+    const int nact = NB - j;  // active slots [0, nact); finished slots [nact, NB)
+    // ---- partial sums over this thread's rows, x = pivot column below the
+    // diagonal: active slot s -> x^T a_s (s = 0 gives sigma = ||x||^2),
+    // finished slot of column i -> x^T v_i(rows > j), which with the pivot row
+    // gives the Gram entry v_i^T v_j = v_i[j] + scale x^T v_i(rows > j)
+    double xq[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const long r = rowv[q];
-      vprevq[q] = (r < m && j >= 1) ? ((r > j - 1) ? sel<NB>(a[q], j - 1) : (r == j - 1 ? 1.0 : 0.0)) : 0.0;
-      xq[q] = (r < m && j < k && r > j) ? sel<NB>(a[q], j) : 0.0;
+      xq[q] = (r < m && r > j) ? a[q][0] : 0.0;
     }
 #pragma unroll
-    for (int cb = 0; cb < NB; cb += 16) {
-      double pv[16];
+    for (int cb = 0; cb < NB; cb += CH) {
+      double e[CH];
 #pragma unroll
-      for (int c = 0; c < 16; c++) pv[c] = 0.0;
+      for (int cc = 0; cc < CH; cc++) e[cc] = 0.0;
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
-        const long r = rowv[q];
-        if (r < m) {
-          const double vprev = vprevq[q];
-          const double x = xq[q];
 #pragma unroll
-          for (int cc = 0; cc < 16; cc++) {
-            const int c = cb + cc;
-            double contrib = 0.0;
-            if (c < j - 1)
-              contrib = a[q][c] * vprev;  // r >= j-1 > c wherever vprev != 0
-            else if (c == j)
-              contrib = x * x;
-            else if (c > j && c < nc)
-              contrib = x * a[q][c];
-            pv[cc] += contrib;
-          }
-        }
+        for (int cc = 0; cc < CH; cc++) e[cc] += xq[q] * a[q][cb + cc];
       }
-      double mine = wave_reduce_scatter<16>(pv, lane);
-      if ((lane & 3) == 0) red[wave][cb + (lane >> 2)] = mine;
+      const double mine = wave_reduce_scatter<CH>(e, lane);
+      if ((lane & (64 / CH - 1)) == 0) red[wave][cb + lane / (64 / CH)] = mine;
     }
+    if (g == 0) {
+#pragma unroll
+      for (int q = 0; q < RPT; q++)
+        if (rowv[q] == j) {
+#pragma unroll
+          for (int s = 0; s < NB; s++) piv[s] = a[q][s];
+        }
+    }
+    if (tid == 0) s_ok = 1;
     __syncthreads();
-    // ---- exchange.  One workgroup: straight from LDS.  Several: every group
-    // stores its record (partials, and the pivot row from group 0) into ring
-    // slot j%4 with device-coherent stores; readers poll the values themselves
-    // until none equals the slot's sentinel (a NaN whose payload encodes the
-    // launch epoch and the step).  The sentinel of step j+2 is written at step
-    // j, into the slot that held step j-2, which every reader has consumed by
-    // then; the vmcnt wait of this step's polling orders it before the values
-    // of step j+1.
-    if (G == 1) {
-      if (tid < NB) tot[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-      if (tid == j && j < k) {
-#pragma unroll
-        for (int c = 0; c < NB; c++) piv[c] = a[0][c];
-      }
-      __syncthreads();
-    } else {
+    // ---- exchange between workgroups.  Every group stores its record (the
+    // partial sums; group 0 also the pivot row) into ring slot j % 4 with
+    // device-coherent stores; readers poll the values until none equals the
+    // slot's sentinel (a NaN whose payload encodes the launch epoch and the
+    // step).  The sentinel of step j + 2 is written at step j into the slot
+    // that held step j - 2, which every reader consumed before it could write
+    // its record of step j - 1; the wait before the next record orders the
+    // sentinel ahead of it.
+    if (G > 1) {
       const int slot = j & 3;
-      const long rec = 2 * NB;
-      double *myrec = part + ((long)slot * G + g) * rec;
-      if (tid < NB) st_dev(&myrec[tid], red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
-      if (g == 0 && tid == j && j < k) {
-#pragma unroll
-        for (int c = 0; c < NB; c++) st_dev(&myrec[NB + c], a[0][c]);
-      }
-      // sentinel for step j+2 into its slot (this group's record only)
-      {
-        const unsigned long long snt = 0x7FF8000000000000ull | ((unsigned long long)(epoch & 0xFFFFFu) << 24) |
-                                       (unsigned long long)((j + 2) & 0xFFFFFF);
+      wait_mem();
+      if (tid < NB) {
+        double *myrec = part + ((long)slot * G + g) * rec;
+        st_dev(&myrec[tid], red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
+        if (g == 0) st_dev(&myrec[NB + tid], piv[tid]);
+        const double snt = __longlong_as_double((long long)panel_sentinel(epoch, j + 2));
         double *nrec = part + ((long)((j + 2) & 3) * G + g) * rec;
-        for (int c = tid; c < (int)rec; c += 256) st_dev(&nrec[c], __longlong_as_double((long long)snt));
+        st_dev(&nrec[tid], snt);
+        st_dev(&nrec[NB + tid], snt);
       }
-      const unsigned long long want = 0x7FF8000000000000ull | ((unsigned long long)(epoch & 0xFFFFFu) << 24) |
-                                      (unsigned long long)(j & 0xFFFFFF);
-      if (tid == 0) s_ok = 1;
-      if (tid < NB) tot[tid] = 0.0;
-      for (int g0 = 0; g0 < G; g0 += 32) {
-        const int gn = min(32, G - g0);
-        __syncthreads();
-        for (int idx = tid; idx < gn * NB; idx += 256) {
-          const double *src = part + ((long)slot * G + g0 + idx / NB) * rec + (idx % NB);
-          double v = ld_dev(src);
-          const unsigned long long t0 = clock100();
-          while ((unsigned long long)__double_as_longlong(v) == want) {
-            __builtin_amdgcn_s_sleep(1);
-            v = ld_dev(src);
-            if (clock100() - t0 > kWatchdogTicks) {
-              atomicOr(err, (unsigned)EVD_E_TIMEOUT);
-              s_ok = 0;
-              break;
+      const unsigned long long want = panel_sentinel(epoch, j);
+      double tacc = 0.0, pacc = 0.0;
+      for (int g0 = 0; g0 < G; g0 += PASSG) {
+        const int gn = min(PASSG, G - g0);
+        const int items = gn * NB + (g0 == 0 ? NB : 0);
+        for (int base = 0; base < items; base += 4 * 256) {
+          double v[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int idx = base + u * 256 + tid;
+            const int gg = idx / NB, s = idx - gg * NB;
+            const double *src = (gg < gn) ? part + ((long)slot * G + g0 + gg) * rec + s
+                                          : part + ((long)slot * G) * rec + NB + s;
+            v[u] = idx < items ? ld_dev(src) : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int idx = base + u * 256 + tid;
+            if (idx < items) {
+              const int gg = idx / NB, s = idx - gg * NB;
+              const double *src = (gg < gn) ? part + ((long)slot * G + g0 + gg) * rec + s
+                                            : part + ((long)slot * G) * rec + NB + s;
+              const unsigned long long t0 = clock100();
+              while ((unsigned long long)__double_as_longlong(v[u]) == want) {
+                __builtin_amdgcn_s_sleep(1);
+                v[u] = ld_dev(src);
+                if (clock100() - t0 > kWatchdogTicks) {
+                  atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+                  s_ok = 0;
+                  break;
+                }
+              }
+              gpart[idx] = v[u];
             }
           }
-          gpart[idx] = v;
         }
         __syncthreads();
         if (tid < NB) {
-          double sacc = tot[tid];
-          for (int gg = 0; gg < gn; gg++) sacc += gpart[gg * NB + tid];
-          tot[tid] = sacc;
+          for (int gg = 0; gg < gn; gg++) tacc += gpart[gg * NB + tid];
+          if (g0 == 0) pacc = gpart[gn * NB + tid];
         }
+        __syncthreads();
       }
-      if (j < k && tid < NB) {
-        const double *src = part + ((long)slot * G + 0) * rec + NB + tid;
-        double v = ld_dev(src);
-        const unsigned long long t0 = clock100();
-        while ((unsigned long long)__double_as_longlong(v) == want) {
-          __builtin_amdgcn_s_sleep(1);
-          v = ld_dev(src);
-          if (clock100() - t0 > kWatchdogTicks) {
-            atomicOr(err, (unsigned)EVD_E_TIMEOUT);
-            s_ok = 0;
-            break;
-          }
-        }
-        piv[tid] = v;
+      if (tid < NB) {
+        tot[tid] = tacc;
+        piv[tid] = pacc;
       }
       __syncthreads();
       if (!s_ok) return;
     }
-    __syncthreads();
-    if (tid < NB && tid < j - 1) gram[tid][j - 1] = tot[tid];
-    if (j == k) break;
-    if (tid == 0) {
-      const double alpha = piv[j];
-      const double sigma = tot[j];
+    // ---- reflector (LAPACK dlarfg), w_s = tau v^T a_s, Gram column j
+    if (tid < NB) {
+      const int s = tid;
+      double ts, t0;
+      if (G == 1) {
+        ts = red[0][s] + red[1][s] + red[2][s] + red[3][s];
+        t0 = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+      } else {
+        ts = tot[s];
+        t0 = tot[0];
+      }
+      const double alpha = piv[0], sigma = t0;
       double beta, tau, scale;
       if (sigma == 0.0) {
         beta = alpha;
         tau = 0.0;
         scale = 0.0;
       } else {
-        double nrm = sqrt(alpha * alpha + sigma);
+        const double nrm = sqrt(alpha * alpha + sigma);
         beta = alpha >= 0.0 ? -nrm : nrm;
         tau = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
       }
-      sc[0] = beta;
-      sc[1] = tau;
-      sc[2] = scale;
-      taus[j] = tau;
+      wv[s] = (s >= 1 && s < nact) ? tau * (piv[s] + scale * ts) : 0.0;
+      if (s >= nact) gram[s - nact][j] = piv[s] + scale * ts;
+      if (s == 0) {
+        sc[0] = beta;
+        sc[1] = scale;
+        taus[j] = tau;
+      }
     }
     __syncthreads();
-    if (tid < NB) {
-      const int c = tid;
-      wv[c] = (c > j && c < nc) ? sc[1] * (piv[c] + tot[c] * sc[2]) : 0.0;
-    }
-    __syncthreads();
-    const double beta = sc[0], scale = sc[2];
+    // ---- apply H_j to the trailing columns and rotate the slots
+    const double beta = sc[0], scale = sc[1];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const long r = rowv[q];
-      if (r >= m) continue;
-      if (r > j) {
-        const double v = sel<NB>(a[q], j) * scale;
+      const double x = a[q][0];
+      const double v = (r > j) ? x * scale : (r == j ? 1.0 : 0.0);
+      const double fin = (r > j) ? v : (r == j ? beta : x);
 #pragma unroll
-        for (int c = 0; c < NB; c++) {
-          if (c == j)
-            a[q][c] = v;
-          else if (c > j)
-            a[q][c] -= v * wv[c];
-        }
-      } else if (r == j) {
-#pragma unroll
-        for (int c = 0; c < NB; c++) {
-          if (c == j)
-            a[q][c] = beta;
-          else if (c > j)
-            a[q][c] -= wv[c];
-        }
-      }
+      for (int s = 0; s < NB - 1; s++) a[q][s] = a[q][s + 1] - v * wv[s + 1];
+      a[q][NB - 1] = fin;
     }
   }
+  // columns back to natural order (column c in slot c)
+  for (int t = k; t < NB; t++) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const double first = a[q][0];
+#pragma unroll
+      for (int s = 0; s < NB - 1; s++) a[q][s] = a[q][s + 1];
+      a[q][NB - 1] = first;
+    }
+  }
+  __syncthreads();
 
-  // ---- T by the dlarft forward/columnwise recurrence (all workgroups alike)
-  for (int j = 0; j < k; j++) {
-    if (tid < j) {
+  // ---- T by the dlarft forward/columnwise recurrence, one row per thread:
+  // T[i][j] = -tau_j sum_{l=i}^{j-1} T[i][l] (v_l^T v_j)
+  if (tid < k) {
+    const int i = tid;
+    Tt[i][i] = taus[i];
+    for (int jj = i + 1; jj < k; jj++) {
       double s = 0.0;
-      for (int l = tid; l < j; l++) s += T[tid][l] * gram[l][j];
-      T[tid][j] = -taus[j] * s;
+      for (int l = i; l < jj; l++) s += Tt[l][i] * gram[l][jj];
+      Tt[jj][i] = -taus[jj] * s;
     }
-    if (tid == j) T[j][j] = taus[j];
-    __syncthreads();
+    if (g == 0 && tau_out) tau_out[i] = taus[i];
   }
-  if (g == 0 && tau_out && tid < k) tau_out[tid] = taus[tid];
+  __syncthreads();
 
-  // ---- outputs: R / [R;0] first, then a[] becomes V in place, then W = V T
+  // ---- outputs: R / [R;0], then a[] becomes V in place, Y = V, W = V T
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     const long r = rowv[q];
-    if (r >= m) continue;
+    if (r < m) {
 #pragma unroll
-    for (int c = 0; c < NB; c++) {
-      if (c >= nc) continue;
-      const double rv = (r <= c) ? a[q][c] : 0.0;
-      if (R && r < nc) R[r + (long)c * ldr] = (float)rv;
-      if (Aout) Aout[r + (long)c * ldo] = (float)rv;
+      for (int c = 0; c < NB; c++) {
+        if (c >= nc) continue;
+        const double rv = (r <= c) ? a[q][c] : 0.0;
+        if (R && r < nc) R[r + (long)c * ldr] = (float)rv;
+        if (Aout) Aout[r + (long)c * ldo] = (float)rv;
+      }
     }
 #pragma unroll
     for (int c = 0; c < NB; c++) a[q][c] = (c < k) ? ((r > c) ? a[q][c] : (r == c ? 1.0 : 0.0)) : 0.0;
+    if (r < m && Y) {
 #pragma unroll
-    for (int c = 0; c < NB; c++) {
-      if (c >= nc) continue;
-      double w = 0.0;
+      for (int c = 0; c < NB; c++)
+        if (c < nc) Y[r + (long)c * ldy] = (float)a[q][c];
+    }
+  }
+  if (W) {
+    for (int c = 0; c < nc; c++) {
 #pragma unroll
-      for (int l = 0; l <= c; l++) w += a[q][l] * T[l][c];
-      if (W) W[r + (long)c * ldw] = (float)(c < k ? w : 0.0);
-      if (Y) Y[r + (long)c * ldy] = (float)a[q][c];
+      for (int q = 0; q < RPT; q++) {
+        const long r = rowv[q];
+        double w = 0.0;
+#pragma unroll
+        for (int l = 0; l < NB; l++) w += a[q][l] * Tt[c][l];
+        if (r < m) W[r + (long)c * ldw] = (float)w;
+      }
     }
   }
 }
@@ -338,8 +351,7 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
   if (G > maxres) return set_error(ctx, EVD_E_UNSUPPORTED, "panel of %ld rows needs %d co-resident groups (max %d)", m, G, maxres);
   ArenaFrame frame(ctx);
   double *part = arena_alloc_n<double>(ctx, (size_t)4 * G * 2 * NB);
-  double *piv = arena_alloc_n<double>(ctx, (size_t)2 * NB);
-  if (!part || !piv) return EVD_E_NOMEM;
+  if (!part) return EVD_E_NOMEM;
   if (G > kTagWords) return set_error(ctx, EVD_E_UNSUPPORTED, "panel needs %d exchange tags", G);
   unsigned *tags = ctx->tags;
   // tags carry a launch epoch, so stale tags of earlier launches never match
@@ -347,7 +359,7 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
   if (epoch == 0) epoch = (++ctx->epoch) & 0x1FFFFFFu;
   unsigned *err = ctx->ctrl + kCtrlError;
   int mi = (int)m, ni = (int)nc;
-  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &piv, &tau_out, &tags, &epoch, &err};
+  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &tau_out, &tags, &epoch, &err};
   EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0,
                                           ctx->stream));
   return 0;
