@@ -10,12 +10,20 @@
 //   N <- N H (block below S, which becomes the next B1 - the bulge).
 // u is stored in U[p .. p+len-1, s] ("noBandU" layout, FP32).
 //
-// MI355X design: persistent cooperative grid; workgroup g runs sweeps
-// g, g+G, ...; the three b x b windows (B1, S, N) live in LDS in FP64; the
-// block below (N) stays in LDS as the next step's B1, so each step reads S
-// and N and writes B1 and S.  Sweeps are ordered through per-sweep progress
-// flags in HBM (release/acquire at agent scope, no grid barrier): sweep s may
-// run its step at row p once sweep s-1 has published a next-row >= p + 2b.
+// MI355X design: sweeps are handed out by an atomic ticket in increasing
+// order, so a sweep's predecessor is always owned by a running workgroup and
+// no co-residency (cooperative launch) is needed.  The band lives in HBM and
+// every access of the chase is device-coherent (sc1 loads and stores, which
+// bypass the non-coherent per-XCD caches; on this chip the L1-bypassing sc0
+// loads do not observe other CUs' stores, so an XCD-local exchange is not
+// available), and a vmcnt(0) wait orders a step's stores before its flag.
+// Per step the S and N windows are held in registers (a row per 256/BW
+// lanes), B1 and the next B1 in LDS; H B1 is applied by one wave while the
+// others wait for their window loads.  Step (s, p) needs step (s-1, p-1+2b)
+// only for one element, B1[0][0] of that step (row p+2b-1, column p+b-1),
+// which becomes beta; the owner publishes that beta at the end of its
+// previous step, so sweep s may start row p as soon as sweep s-1 has
+// published a next-row >= p + 2b - 1 (a lag of two steps, not three).
 // Every wait is bounded by a watchdog so the grid always drains.
 #include <climits>
 
@@ -23,150 +31,187 @@
 
 namespace evd {
 
+constexpr unsigned kBufferWord3 = 0x00020000u;  // raw buffer descriptor: 32-bit data format
+
+constexpr int kSc1 = 16;  // buffer-op cache policy: sc1, device-coherent
+
+__device__ inline double ld_band(__amdgpu_buffer_rsrc_t r, unsigned off_elems) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off_elems * 8u, 0, kSc1));
+}
+__device__ inline void st_band(__amdgpu_buffer_rsrc_t r, unsigned off_elems, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((__vector_size__(2 * sizeof(unsigned)))) unsigned, v),
+                                        r, off_elems * 8u, 0, kSc1);
+}
+
+// LAPACK-style reflector of the chase (reference sign convention): from the
+// column x = Bc[0..lenx) (LDS, stride LD), u = (x/||x|| + sign(x0) e1) /
+// sqrt(|x0/||x|| + sign(x0)|), ||u||^2 = 2, H x = beta e1.  Wave 0 only.
 template <int BW>
-__global__ __launch_bounds__(256) void chase_kernel(int n, int b, double *__restrict__ band, long ldb,
-                                                    float *__restrict__ U, long ldu, int *__restrict__ com,
+__device__ inline void chase_reflector(const double *Bc, int lenx, int lane, double *u, double *s_beta, float *U,
+                                       long urow, long ucol) {
+  constexpr int LD = BW + 1;
+  const double x = (lane < lenx) ? Bc[lane * LD] : 0.0;
+  const double nrm = sqrt(wave_sum(x * x));
+  const double x0 = Bc[0];
+  double uu = 0.0;
+  if (nrm > 0.0) {
+    const double inv = 1.0 / nrm;
+    const double u0 = x0 * inv + (x0 >= 0.0 ? 1.0 : -1.0);
+    const double scl = 1.0 / sqrt(fabs(u0));
+    uu = (lane == 0 ? u0 : x * inv) * scl;
+  }
+  if (lane < BW) u[lane] = (lane < lenx) ? uu : 0.0;
+  if (lane < lenx) U[(urow + lane) + ucol] = (float)uu;
+  if (lane == 0) *s_beta = (nrm > 0.0) ? (x0 >= 0.0 ? -nrm : nrm) : x0;
+}
+
+template <int BW>
+__global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int b, double *__restrict__ band, long ldb,
+                                                    unsigned band_bytes, float *__restrict__ U, long ldu,
+                                                    int *__restrict__ com, unsigned *__restrict__ ticket,
                                                     unsigned *__restrict__ err) {
   constexpr int LD = BW + 1;
-  __shared__ double sA[BW * LD], sBuf[BW * LD], sS[BW * LD];
-  __shared__ double u[BW], tv[BW], yv[BW], qv[BW], wv[BW];
+  constexpr int LPR = 256 / BW;  // lanes per window row
+  constexpr int EPT = BW / LPR;  // window columns per lane
+  constexpr int HALVES = 64 / BW;
+  __shared__ double sB[2][BW * LD];
+  __shared__ double u[BW], yv[BW], qv[BW];
   __shared__ double s_beta;
-  __shared__ int s_ok;
-  const int G = gridDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ int s_sweep, s_ok;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ri = tid / LPR, kc = tid % LPR;
   const int nsweeps = n - 2;
-  double *B1 = sA, *Nb = sBuf;
-  double *S = sS;
+  const unsigned ld = (unsigned)ldb;  // band offsets fit 32 bits (checked by the launcher)
 
-  for (int s = blockIdx.x; s < nsweeps; s += G) {
+  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(band, 0, (int)band_bytes, kBufferWord3);
+
+  // tid 0 polls the predecessor's progress word; false = watchdog expired
+  auto wait_prev = [&](int s, int need) -> bool {
+    if (tid == 0) {
+      s_ok = 1;
+      if (s > 0) {
+        const unsigned long long t0 = clock100();
+        while (ld_dev(&com[s - 1]) < need) {
+          __builtin_amdgcn_s_sleep(1);
+          if (clock100() - t0 > kWatchdogTicks) {
+            atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+            s_ok = 0;
+            break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    return s_ok != 0;
+  };
+
+  while (true) {
+    if (tid == 0)
+      s_sweep = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int s = s_sweep;
+    __syncthreads();
+    if (s >= nsweeps) return;
     int p = s + 1;
     int len = min(b, n - p);
     int colB1 = 1, cB1 = s;
     bool first = true;
-    __syncthreads();
-    while (true) {
-      // ---- wait until sweep s-1 is 2b rows ahead
-      if (tid == 0) {
-        s_ok = 1;
-        if (s > 0) {
-          const int need = p + 2 * b;
-          unsigned long long t0 = clock100();
-          while (ld_dev(&com[s - 1]) < need) {
-            __builtin_amdgcn_s_sleep(1);
-            if (clock100() - t0 > kWatchdogTicks) {
-              atomicOr(err, (unsigned)EVD_E_TIMEOUT);
-              s_ok = 0;
-              break;
+    double *B1 = sB[0], *Bn = sB[1];
+    bool ok = wait_prev(s, p + 2 * b - 1);
+    if (ok) {
+      // column s below the diagonal is final: it becomes B1[:, 0]
+      for (int i = tid; i < BW; i += 256) B1[i * LD] = (i < len) ? ld_band(rb, (unsigned)(1 + i) + (unsigned)s * ld) : 0.0;
+      __syncthreads();
+      if (wave == 0) chase_reflector<BW>(B1, len, lane, u, &s_beta, U, p, (long)s * ldu);
+      __syncthreads();
+    }
+    while (ok) {
+      const int len2 = max(0, min(b, n - p - len));
+      const bool last = (len2 <= 1);
+      // ---- S (symmetric, from the lower band) and N windows into registers
+      double Sr[EPT], Nr[EPT];
+#pragma unroll
+      for (int r = 0; r < EPT; r++) {
+        const int kk = kc + LPR * r;
+        const int lo = min(ri, kk), dd = ri > kk ? ri - kk : kk - ri;
+        Sr[r] = (ri < len && kk < len) ? ld_band(rb, (unsigned)dd + (unsigned)(p + lo) * ld) : 0.0;
+        Nr[r] = (ri < len2 && kk < len) ? ld_band(rb, (unsigned)(len + ri - kk) + (unsigned)(p + kk) * ld) : 0.0;
+      }
+      // ---- B1 <- H B1 and write-back (wave 3, while the loads are in flight)
+      if (wave == 3) {
+        const int c = lane % BW, h = lane / BW;
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < BW; i += HALVES) t += u[i + h] * B1[(i + h) * LD + c];
+        if (HALVES == 2) t += __shfl_xor(t, 32, 64);
+        if (c < colB1) {
+          const double beta = s_beta;
+          const int col = cB1 + c;
+#pragma unroll
+          for (int i = 0; i < BW; i += HALVES) {
+            const int ii = i + h;
+            if (ii < len && !(c == 0 && ii == 0 && !first)) {
+              const double v = (c == 0) ? (ii == 0 ? beta : 0.0) : B1[ii * LD + c] - u[ii] * t;
+              st_band(rb, (unsigned)(p + ii - col) + (unsigned)col * ld, v);
             }
           }
         }
       }
-      __syncthreads();
-      if (!s_ok) {
-        if (tid == 0) st_dev(&com[s], INT_MAX);
-        return;
+      // ---- y = S u and q = N u (row sums over the LPR lanes of a row)
+      double yp = 0.0, qp = 0.0;
+#pragma unroll
+      for (int r = 0; r < EPT; r++) {
+        const double uk = u[kc + LPR * r];
+        yp += Sr[r] * uk;
+        qp += Nr[r] * uk;
       }
-      if (first) {
-        // column s is final once sweep s-1 has passed row s+1+2b
-        for (int i = tid; i < len; i += 256) B1[i * LD + 0] = ld_dev(&band[(p + i - s) + (long)s * ldb]);
-        first = false;
+#pragma unroll
+      for (int o = LPR / 2; o > 0; o >>= 1) {
+        yp += __shfl_xor(yp, o, 64);
+        qp += __shfl_xor(qp, o, 64);
       }
-      const int len2 = max(0, min(b, n - p - len));
-      // ---- load S (lower -> full) and N
-      for (int idx = tid; idx < len * len; idx += 256) {
-        int i = idx % len, kk = idx / len;
-        if (i >= kk) {
-          double v = ld_dev(&band[(i - kk) + (long)(p + kk) * ldb]);
-          S[i * LD + kk] = v;
-          S[kk * LD + i] = v;
-        }
-      }
-      for (int idx = tid; idx < len2 * len; idx += 256) {
-        int i = idx % len2, kk = idx / len2;
-        Nb[i * LD + kk] = ld_dev(&band[(len + i - kk) + (long)(p + kk) * ldb]);
+      if (kc == 0) {
+        yv[ri] = yp;
+        qv[ri] = qp;
       }
       __syncthreads();
-      // ---- reflector from B1[:, 0]
-      if (wave == 0) {
-        double x = (lane < len) ? B1[lane * LD] : 0.0;
-        double nrm = sqrt(wave_sum(x * x));
-        double x0 = B1[0];
-        double uu = 0.0;
-        if (nrm > 0.0) {
-          double inv = 1.0 / nrm;
-          double u0 = x0 * inv + (x0 >= 0.0 ? 1.0 : -1.0);
-          double scl = 1.0 / sqrt(fabs(u0));
-          uu = (lane == 0 ? u0 : x * inv) * scl;
-        }
-        if (lane < BW) u[lane] = (lane < len) ? uu : 0.0;
-        if (lane < len) U[(p + lane) + (long)s * ldu] = (float)uu;
-        if (lane == 0) s_beta = (nrm > 0.0) ? (x0 >= 0.0 ? -nrm : nrm) : x0;
+      // ---- S <- H S H (rank-2 with w = y - (u^T y / 2) u), N <- N H
+      const double psi = wave_sum((lane < BW) ? u[lane] * yv[lane] : 0.0);
+      const double ui = u[ri], wi = yv[ri] - 0.5 * psi * ui, qi = qv[ri];
+#pragma unroll
+      for (int r = 0; r < EPT; r++) {
+        const int kk = kc + LPR * r;
+        const double uk = u[kk], wk = yv[kk] - 0.5 * psi * uk;
+        Sr[r] -= ui * wk + wi * uk;
+        Nr[r] -= qi * uk;
+        if (ri >= kk && ri < len && kk < len) st_band(rb, (unsigned)(ri - kk) + (unsigned)(p + kk) * ld, Sr[r]);
+        Bn[ri * LD + kk] = Nr[r];
+        if (last && ri < len2 && kk < len) st_band(rb, (unsigned)(len + ri - kk) + (unsigned)(p + kk) * ld, Nr[r]);
       }
       __syncthreads();
-      // ---- matrix-vector products: t = B1^T u, y = S u, q = N u
-      if (wave == 0 && lane < colB1) {
-        double acc = 0.0;
-        for (int i = 0; i < len; i++) acc += u[i] * B1[i * LD + lane];
-        tv[lane] = acc;
-      } else if (wave == 1 && lane < len) {
-        double acc = 0.0;
-        for (int kk = 0; kk < len; kk++) acc += S[lane * LD + kk] * u[kk];
-        yv[lane] = acc;
-      } else if (wave == 2 && lane < len2) {
-        double acc = 0.0;
-        for (int kk = 0; kk < len; kk++) acc += Nb[lane * LD + kk] * u[kk];
-        qv[lane] = acc;
-      }
-      __syncthreads();
-      if (wave == 0) {
-        double pr = (lane < len) ? u[lane] * yv[lane] : 0.0;
-        double psi = wave_sum(pr);
-        if (lane < len) wv[lane] = yv[lane] - 0.5 * psi * u[lane];
-      }
-      __syncthreads();
-      // ---- rank-1 / rank-2 updates
-      for (int idx = tid; idx < len * colB1; idx += 256) {
-        int i = idx % len, c = idx / len;
-        if (c == 0)
-          B1[i * LD] = (i == 0) ? s_beta : 0.0;
-        else
-          B1[i * LD + c] -= u[i] * tv[c];
-      }
-      for (int idx = tid; idx < len * len; idx += 256) {
-        int i = idx % len, kk = idx / len;
-        S[i * LD + kk] -= u[i] * wv[kk] + wv[i] * u[kk];
-      }
-      for (int idx = tid; idx < len2 * len; idx += 256) {
-        int i = idx % len2, kk = idx / len2;
-        Nb[i * LD + kk] -= qv[i] * u[kk];
-      }
-      __syncthreads();
-      // ---- write back B1 and S (and N at the end of the sweep)
-      for (int idx = tid; idx < len * colB1; idx += 256) {
-        int i = idx % len, c = idx / len;
-        st_dev(&band[(p + i - (cB1 + c)) + (long)(cB1 + c) * ldb], B1[i * LD + c]);
-      }
-      for (int idx = tid; idx < len * len; idx += 256) {
-        int i = idx % len, kk = idx / len;
-        if (i >= kk) st_dev(&band[(i - kk) + (long)(p + kk) * ldb], S[i * LD + kk]);
-      }
-      const bool last = (len2 <= 1);
-      if (last) {
-        for (int idx = tid; idx < len2 * len; idx += 256) {
-          int i = idx % len2, kk = idx / len2;
-          st_dev(&band[(len + i - kk) + (long)(p + kk) * ldb], Nb[i * LD + kk]);
-        }
+      // ---- next reflector from N[:, 0]; its beta (row p+len, column p) is
+      // published now, before the progress word, for the successor sweep
+      if (!last && wave == 0) {
+        chase_reflector<BW>(Bn, len2, lane, u, &s_beta, U, p + len, (long)s * ldu);
+        if (lane == 0) st_band(rb, (unsigned)len + (unsigned)p * ld, s_beta);
       }
       wait_mem();
       __syncthreads();
       if (tid == 0) st_dev(&com[s], last ? INT_MAX : p + len);
       if (last) break;
       double *t = B1;
-      B1 = Nb;
-      Nb = t;
+      B1 = Bn;
+      Bn = t;
       colB1 = len;
       cB1 = p;
       p += len;
       len = len2;
+      first = false;
+      ok = wait_prev(s, p + 2 * b - 1);
+    }
+    if (!ok) {
+      if (tid == 0) st_dev(&com[s], INT_MAX);
+      return;
     }
   }
 }
@@ -200,16 +245,21 @@ int band_pack(evd_ctx *ctx, long n, long b, const float *A, long lda, double *ba
 }
 
 template <int BW>
-static int launch_chase(evd_ctx *ctx, long n, long b, double *band, long ldb, float *U, long ldu, int *com) {
-  int maxres = 0;
-  EVD_TRY(max_coresident_blocks(ctx, (const void *)chase_kernel<BW>, 256, 0, &maxres));
-  long nsweeps = n - 2;
-  int G = ctx->tune.chase_groups > 0 ? ctx->tune.chase_groups : ctx->num_cus;
-  G = (int)std::min<long>(std::min<long>(G, maxres), std::max<long>(1, nsweeps));
+static int launch_chase(evd_ctx *ctx, long n, long b, double *band, long ldb, float *U, long ldu, int *com,
+                        unsigned *ticket) {
+  const long bytes = ldb * n * (long)sizeof(double);
+  if (bytes >= (1L << 31))
+    return set_error(ctx, EVD_E_UNSUPPORTED, "band of %ld bytes exceeds the 2 GiB buffer range of the chase", bytes);
+  // workgroups: by default as many as are resident (more only queue)
+  int per_cu = 0;
+  EVD_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)chase_kernel<BW>, 256, 0));
+  long G = ctx->tune.chase_groups > 0 ? ctx->tune.chase_groups : (long)std::max(1, per_cu) * ctx->num_cus;
+  G = std::min<long>(G, std::max<long>(1, n - 2));
   int ni = (int)n, bi = (int)b;
+  unsigned nbytes = (unsigned)bytes;
   unsigned *err = ctx->ctrl + kCtrlError;
-  void *args[] = {&ni, &bi, &band, &ldb, &U, &ldu, &com, &err};
-  EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)chase_kernel<BW>, dim3(G), dim3(256), args, 0, ctx->stream));
+  chase_kernel<BW><<<(unsigned)G, 256, 0, ctx->stream>>>(ni, bi, band, ldb, nbytes, U, ldu, com, ticket, err);
+  EVD_LAUNCHED(ctx);
   return 0;
 }
 
@@ -218,13 +268,15 @@ int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, lon
   EVD_TRY(fill_f32(ctx, U, ldu, n, ldu, 0.f));
   if (n > 2) {
     ArenaFrame frame(ctx);
-    int *com = arena_alloc_n<int>(ctx, (size_t)n);
+    // progress words com[0..n), then the sweep ticket
+    int *com = arena_alloc_n<int>(ctx, (size_t)n + 1);
     if (!com) return EVD_E_NOMEM;
-    EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), (size_t)n, 0u));
+    unsigned *ticket = reinterpret_cast<unsigned *>(com + n);
+    EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), (size_t)n + 1, 0u));
     if (b <= 32)
-      EVD_TRY(launch_chase<32>(ctx, n, b, band, ldband, U, ldu, com));
+      EVD_TRY(launch_chase<32>(ctx, n, b, band, ldband, U, ldu, com, ticket));
     else if (b <= 64)
-      EVD_TRY(launch_chase<64>(ctx, n, b, band, ldband, U, ldu, com));
+      EVD_TRY(launch_chase<64>(ctx, n, b, band, ldband, U, ldu, com, ticket));
     else
       return set_error(ctx, EVD_E_UNSUPPORTED, "bandwidth %ld > 64", b);
   }
@@ -254,7 +306,7 @@ extern "C" int evd_sb2st_f64(evd_ctx *ctx, long n, long b, double *band, long ld
   EVD_ARG(ctx, b >= 1 && b <= 64, 3);
   EVD_ARG(ctx, ldband >= 2 * b, 5);
   EVD_ARG(ctx, ldu >= n + b, 7);
-  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)n * sizeof(int) + 4096));
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)(n + 2) * sizeof(int) + 4096));
   EVD_TRY(sb2st(ctx, n, b, band, ldband, U, ldu, D, E));
   return check_device_error(ctx, "evd_sb2st_f64");
 }

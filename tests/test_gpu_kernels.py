@@ -120,7 +120,7 @@ def test_sy2sb_band_spectrum(ctx, n, b, nb):
     assert np.abs(Q.T @ A.astype(np.float64) @ Q - B).max() < 1e-5 * nrm
 
 
-@pytest.mark.parametrize("n,b", [(40, 32), (130, 32), (600, 32), (300, 64), (1500, 32)])
+@pytest.mark.parametrize("n,b", [(40, 32), (130, 32), (600, 32), (300, 64), (1500, 32), (200, 20), (257, 48), (3, 32)])
 def test_sb2st_matches_oracle(ctx, n, b):
     import evd4sc2025_amd as E
 
