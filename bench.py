@@ -26,6 +26,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp32_mfma_tflops": 157.3, "fp64_tflops": 78.6, "hbm_gbs": 8000.0}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bcback_chase_n16384_hbm_traffic.md")
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` at the headline size, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("| `evd::" + kernel):
+                return float(line.strip().strip("|").split("|")[-1]) * 1e9
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(n: int = 4096, seed: int = 0):
@@ -110,9 +123,20 @@ def main():
     bc_ms = stage_ms["bc_back"]
     bc_flops = 2.0 * n ** 3
     achieved = bc_flops / (bc_ms * 1e-3) / 1e12
+    traffic = profiled_traffic("bcback_kernel") if (n, b) == (16384, 32) else None
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": None, "kernel": "bcback_kernel",
-                "note": "FP32 peak (157.3 TF); the kernel runs on the VALU (no packed FP32)"}
+                "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": traffic, "kernel": "bcback_kernel",
+                "note": "2 n^3 flops per launch over the hipEvent time of the launch; FP32 peak 157.3 TF (the kernel "
+                        "runs FP32 FMA on the VALU); traffic = copy-calibrated FETCH_SIZE + WRITE_SIZE bytes per launch "
+                        "from the committed rocprofv3 summary"}
+    # the bulge chase against HBM: (1.5 b + 1) n^2 FP64 window bytes moved per launch
+    bc_bytes = (1.5 * b + 1) * n * n * 8
+    chase = {"bound": "hbm", "achieved": bc_bytes / (stage_ms["bc"] * 1e-3) / 1e9 if stage_ms["bc"] else None,
+             "peak": PEAK["hbm_gbs"], "unit": "GB/s", "kernel": "chase_kernel",
+             "us_per_dependent_step": stage_ms["bc"] * 1e3 / (2.0 * n) if stage_ms["bc"] else None,
+             "traffic": profiled_traffic("chase_kernel") if (n, b) == (16384, 32) else None}
+    if chase["achieved"] is not None:
+        chase["frac"] = chase["achieved"] / PEAK["hbm_gbs"]
     stage_tflops = {
         "sbr": (4.0 / 3.0) * n ** 3 / (stage_ms["sbr"] * 1e-3) / 1e12 if stage_ms["sbr"] else None,
         "bc_back": achieved,
@@ -149,6 +173,7 @@ def main():
         "stage_ms": stage_ms,
         "stage_tflops": stage_tflops,
         "roofline": roofline,
+        "chase_roofline": chase,
         "cpu_baseline": cpu,
         "accuracy": acc,
     }
