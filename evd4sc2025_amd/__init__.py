@@ -26,6 +26,8 @@ __all__ = [
 ]
 
 STAGES = ("sbr", "bc", "dc", "bc_back", "sbr_back")
+# extra timer after the stages: the BC back-transformation kernel alone
+TIMERS = STAGES + ("bc_back_kernel",)
 
 
 def _torch():
@@ -129,9 +131,11 @@ class Context:
         self.call("evd_ctx_enable_timing", 1 if on else 0)
 
     def stage_ms(self) -> dict:
-        arr = (C.c_float * len(STAGES))()
-        self.call("evd_ctx_stage_ms", arr, len(STAGES))
-        return {k: float(v) for k, v in zip(STAGES, arr)}
+        """Per-stage device ms of the last syevd (hipEvents on the context
+        stream), plus 'bc_back_kernel': the BC back-transformation kernel."""
+        arr = (C.c_float * len(TIMERS))()
+        self.call("evd_ctx_stage_ms", arr, len(TIMERS))
+        return {k: float(v) for k, v in zip(TIMERS, arr)}
 
     def synchronize(self):
         self.call("evd_ctx_synchronize")

@@ -18,72 +18,219 @@
 // form a pipeline over NW consecutive sweep blocks: wave k lags wave k-1 by one
 // step and receives the b rows wave k-1 has finished through an LDS double
 // buffer, so the column block is streamed from HBM once per NW*BS sweeps and
-// each SIMD runs one wave of pure FMA work.
+// each SIMD runs FMA work.  Rows enter and leave HBM through an LDS transpose:
+// a wave instruction moves 2 columns x 32 consecutive rows of the
+// column-major block (a lane-per-column access would touch 64 cache lines
+// with 4 bytes each and leave the L2 as partial-line writes).
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace evd {
 
-// Fast path: every reflector of the block exists and lies inside the matrix,
-// so the entries are used straight from SGPRs as FMA operands.
-template <int NB, int BS, int W, bool REV, int TT>
-__device__ __attribute__((always_inline)) inline void apply_block_full(float (&z)[W], const float *__restrict__ U,
-                                                                       long ldu, int S0, int wb) {
-  if constexpr (TT < BS) {
-    constexpr int t = REV ? (BS - 1 - TT) : TT;
-    const float *up = U + (wb + t) + (long)(S0 + t) * ldu;
-    float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f;
-#pragma unroll
-    for (int i = 0; i < NB; i += 4) {
-      d0 += up[i] * z[t + i];
-      d1 += up[i + 1] * z[t + i + 1];
-      d2 += up[i + 2] * z[t + i + 2];
-      d3 += up[i + 3] * z[t + i + 3];
-    }
-    const float dot = (d0 + d1) + (d2 + d3);
-#pragma unroll
-    for (int i = 0; i < NB; i++) z[t + i] -= dot * up[i];
-    apply_block_full<NB, BS, W, REV, TT + 1>(z, U, ldu, S0, wb);
+// Packed reflector blocks.  Block (S, jj) holds the BS reflectors of sweeps
+// S*BS .. S*BS+BS-1 at step jj as BS x NB floats, entry [t][i] =
+// U[S*BS + 1 + jj*NB + t + i, S*BS + t], and 0 where the sweep or the row does
+// not exist (so every block can be applied without checks: a zero reflector
+// is the identity).  Sweep block S has J(S) = A + 2 - m S steps, A = (n-2)/NB,
+// m = BS/NB, and its blocks start at index prefix(S) = S (A+2) - m S (S-1)/2.
+// In this layout reflector t of a block sits at the compile-time offset t*NB
+// from the block's base, so its entries reach SGPRs with scalar loads at
+// immediate offsets: no per-reflector address registers.
+__host__ __device__ inline long pack_steps(long A, long m, long S) { return A + 2 - m * S; }
+__host__ __device__ inline long pack_prefix(long A, long m, long S) { return S * (A + 2) - m * S * (S - 1) / 2; }
+
+template <int NB, int BS>
+size_t pack_floats(long n) {
+  if (n <= 2) return 0;
+  const long nsweeps = n - 2, nblk = (nsweeps + BS - 1) / BS, A = (n - 2) / NB;
+  return (size_t)pack_prefix(A, BS / NB, nblk) * BS * NB;
+}
+
+template <int NB, int BS>
+__global__ void bc_pack_kernel(int n, const float *__restrict__ U, long ldu, float *__restrict__ P, int nsweeps) {
+  constexpr long m = BS / NB;
+  const long A = (n - 2) / NB;
+  const long S = blockIdx.y;
+  const long base = pack_prefix(A, m, S) * BS * NB;
+  const long cnt = pack_steps(A, m, S) * BS * NB;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < cnt; idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % NB);
+    const int t = (int)((idx / NB) % BS);
+    const long jj = idx / (NB * BS);
+    const long sw = S * BS + t;
+    const long row = sw + 1 + jj * NB + i;
+    P[base + idx] = (sw < nsweeps && row < n) ? U[row + sw * ldu] : 0.f;
   }
 }
 
-// General path (blocks at the matrix edge): a reflector is skipped when its
-// sweep or its start row does not exist.  Rows of U at and below n must be
-// zero (the pipeline's U is zero-filled; the drop-in entry pads a copy), so no
-// per-entry masking is needed.
+// Reflector entries reach the FMAs through DPP row broadcasts.  A reflector
+// of NB entries is held in R = NB/16 VGPRs: lane j of every 16-lane row holds
+// entries R*j .. R*j+R-1 (one vector load per lane), and entry i is read as
+// register i % R broadcast from lane i / R of each row (row_newbcast).  This
+// costs no extra instruction per FMA, returns only NB*4 bytes per row per
+// reflector from the L1, and lets the loads run PF reflectors ahead.
+// v_fmac with a DPP operand is written as inline asm (the compiler does not
+// fold a DPP move into a MAC).  The broadcast source is always a register
+// written by a vector load, never by a VALU instruction, so the DPP read has
+// no VALU-write hazard.
+template <int NB>
+struct URegs {
+  float r[NB / 16];
+};
+
+template <int NB>
+__device__ __attribute__((always_inline)) inline void load_u(URegs<NB> &u, const float *__restrict__ pt, int j) {
+  if constexpr (NB == 32) {
+    const float2 v = reinterpret_cast<const float2 *>(pt)[j];
+    u.r[0] = v.x;
+    u.r[1] = v.y;
+  } else {
+    const float4 v = reinterpret_cast<const float4 *>(pt)[j];
+    u.r[0] = v.x;
+    u.r[1] = v.y;
+    u.r[2] = v.z;
+    u.r[3] = v.w;
+  }
+}
+
+// d = u[I] * x
+template <int NB, int I>
+__device__ __attribute__((always_inline)) inline float mul_bc(const URegs<NB> &u, float x) {
+  constexpr int R = NB / 16;
+  float d;
+  asm("v_mul_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "=v"(d)
+      : "v"(u.r[I % R]), "v"(x), "n"(I / R));
+  return d;
+}
+// acc += u[I] * x
+template <int NB, int I>
+__device__ __attribute__((always_inline)) inline void fma_bc(float &acc, const URegs<NB> &u, float x) {
+  constexpr int R = NB / 16;
+  asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc)
+      : "v"(u.r[I % R]), "v"(x), "n"(I / R));
+}
+// acc -= u[I] * x
+template <int NB, int I>
+__device__ __attribute__((always_inline)) inline void fms_bc(float &acc, const URegs<NB> &u, float x) {
+  constexpr int R = NB / 16;
+  asm("v_fmac_f32_dpp %0, %1, -%2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc)
+      : "v"(u.r[I % R]), "v"(x), "n"(I / R));
+}
+
+template <int NB, int W, int T0, int I>
+__device__ __attribute__((always_inline)) inline void dot_terms(float (&d)[4], const float (&z)[W], const URegs<NB> &u) {
+  if constexpr (I < NB) {
+    if constexpr (I < 4)
+      d[I] = mul_bc<NB, I>(u, z[T0 + I]);
+    else
+      fma_bc<NB, I>(d[I % 4], u, z[T0 + I]);
+    dot_terms<NB, W, T0, I + 1>(d, z, u);
+  }
+}
+
+template <int NB, int W, int T0, int I>
+__device__ __attribute__((always_inline)) inline void update_terms(float (&z)[W], const URegs<NB> &u, float dot) {
+  if constexpr (I < NB) {
+    fms_bc<NB, I>(z[T0 + I], u, dot);
+    update_terms<NB, W, T0, I + 1>(z, u, dot);
+  }
+}
+
+constexpr int kPrefetch = 4;  // reflectors in flight ahead of the one applied
+
+// Apply the BS reflectors of one packed block to the window (template
+// recursion keeps every window index static).  ring[TT % kPrefetch] holds
+// reflector TT; after use it is refilled with reflector TT + kPrefetch.
 template <int NB, int BS, int W, bool REV, int TT>
-__device__ __attribute__((always_inline)) inline void apply_block(float (&z)[W], const float *__restrict__ U,
-                                                                  long ldu, int S0, int wb, int n, int nsweeps) {
+__device__ __attribute__((always_inline)) inline void apply_packed(float (&z)[W], const float *__restrict__ pb,
+                                                                   URegs<NB> (&ring)[kPrefetch], int j) {
   if constexpr (TT < BS) {
     constexpr int t = REV ? (BS - 1 - TT) : TT;
-    if (S0 + t < nsweeps && wb + t < n - 1) {
-      const float *up = U + (wb + t) + (long)(S0 + t) * ldu;
-      float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f;
-#pragma unroll
-      for (int i = 0; i < NB; i += 4) {
-        d0 += up[i] * z[t + i];
-        d1 += up[i + 1] * z[t + i + 1];
-        d2 += up[i + 2] * z[t + i + 2];
-        d3 += up[i + 3] * z[t + i + 3];
-      }
-      const float dot = (d0 + d1) + (d2 + d3);
-#pragma unroll
-      for (int i = 0; i < NB; i++) z[t + i] -= dot * up[i];
+    const URegs<NB> u = ring[TT % kPrefetch];
+    if constexpr (TT + kPrefetch < BS) {
+      constexpr int tn = REV ? (BS - 1 - (TT + kPrefetch)) : (TT + kPrefetch);
+      load_u<NB>(ring[TT % kPrefetch], pb + tn * NB, j);
     }
-    apply_block<NB, BS, W, REV, TT + 1>(z, U, ldu, S0, wb, n, nsweeps);
+    float d[4];
+    dot_terms<NB, W, t, 0>(d, z, u);
+    const float dot = (d[0] + d[1]) + (d[2] + d[3]);
+    update_terms<NB, W, t, 0>(z, u, dot);
+    apply_packed<NB, BS, W, REV, TT + 1>(z, pb, ring, j);
   }
+}
+
+template <int NB, int BS, int W, bool REV>
+__device__ __attribute__((always_inline)) inline void apply_packed_block(float (&z)[W], const float *__restrict__ pb,
+                                                                         int lane) {
+  const int j = lane & 15;
+  URegs<NB> ring[kPrefetch];
+#pragma unroll
+  for (int q = 0; q < kPrefetch; q++) load_u<NB>(ring[q], pb + (REV ? (BS - 1 - q) : q) * NB, j);
+  apply_packed<NB, BS, W, REV, 0>(z, pb, ring, j);
+}
+
+// LDS row stride of a 32-row x 64-column stage (padded against bank conflicts)
+constexpr int kStageLd = 65;
+
+// f(std::integral_constant<int, C>) for C = B, B + 32, ... < E: the window
+// index stays a compile-time constant however large the body is
+template <int B, int E, typename F>
+__device__ __attribute__((always_inline)) inline void for_chunks(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    for_chunks<B + 32, E>(f);
+  }
+}
+
+__device__ __attribute__((always_inline)) inline void lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// 32 rows [row0, row0 + 32) of the wave's 64 columns from Z into stage[r][c];
+// one wave instruction reads 2 columns x 32 consecutive rows.
+__device__ __attribute__((always_inline)) inline void stage_load32(float *stage, const float *__restrict__ Z, long ldz,
+                                                                   int col0, int ncols, int row0, int n, int lane) {
+  const int r = lane & 31, half = lane >> 5;
+  const int gr = row0 + r;
+#pragma unroll 8
+  for (int it = 0; it < 32; it++) {
+    const int c = 2 * it + half, gc = col0 + c;
+    stage[r * kStageLd + c] = (gc < ncols && gr < n) ? Z[(long)gc * ldz + gr] : 0.f;
+  }
+  lds_order();
+}
+
+// stage[r][c] for rows [row0, row0 + 32) back to Z, rows >= n dropped
+__device__ __attribute__((always_inline)) inline void stage_store32(const float *stage, float *__restrict__ Z, long ldz,
+                                                                    int col0, int ncols, int row0, int n, int lane) {
+  lds_order();
+  const int r = lane & 31, half = lane >> 5;
+  const int gr = row0 + r;
+#pragma unroll 8
+  for (int it = 0; it < 32; it++) {
+    const int c = 2 * it + half, gc = col0 + c;
+    if (gc < ncols && gr < n) Z[(long)gc * ldz + gr] = stage[r * kStageLd + c];
+  }
+  lds_order();
 }
 
 template <int NB, int BS, int NW, bool REV>
 __global__ __launch_bounds__(NW * 64) void bcback_kernel(int n, int ncols, float *__restrict__ Z, long ldz,
-                                                        const float *__restrict__ U, long ldu, int nsweeps) {
+                                                        const float *__restrict__ P, int nsweeps) {
+  static_assert(NB == 32 || NB == 64, "window chunks are 32 rows");
   constexpr int W = BS + NB;
-  __shared__ float xch[(NW > 1 ? NW - 1 : 1) * 2 * NB * 64];
+  constexpr int SLOT = NB * kStageLd;
+  // slot pair of wave k: the rows it hands to wave k+1 (double-buffered), and
+  // its staging area for coalesced global loads and stores
+  __shared__ float xch[NW * 2 * SLOT];
   const int lane = threadIdx.x & 63;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int col = blockIdx.x * 64 + lane;
-  const bool colok = col < ncols;
-  float *Zc = Z + (long)(colok ? col : 0) * ldz;
+  const int col0 = blockIdx.x * 64;
   const int nblk = (nsweeps + BS - 1) / BS;
+  constexpr long m = BS / NB;
+  const long A = (n - 2) / NB;
   float z[W];
 
   const int ngroups = (nblk + NW - 1) / NW;
@@ -109,86 +256,96 @@ __global__ __launch_bounds__(NW * 64) void bcback_kernel(int n, int ncols, float
       if (active && e >= 0 && e < J) {
         const int jj = REV ? e : (J - 1 - e);
         const int wb = S0 + 1 + jj * NB;
-        const float *xin = xch + ((k - 1) * 2 + (e & 1)) * NB * 64;
-        float *xout = xch + (k * 2 + (e & 1)) * NB * 64;
-        // ---- bring the window up to date
+        const float *xin = xch + ((k - 1) * 2 + (e & 1)) * SLOT;
+        float *xout = xch + (k * 2 + (e & 1)) * SLOT;
+        // ---- bring the window up to date (global rows staged through xout,
+        // which is only written with this step's hand-off rows at its end)
+        auto from_stage = [&](auto cc) __attribute__((always_inline)) {
+          constexpr int C = decltype(cc)::value;
+#pragma unroll
+          for (int i = 0; i < 32; i++) z[C + i] = xout[i * kStageLd + lane];
+          lds_order();
+        };
         if (REV) {
           if (e == 0) {
             const int ng = (k == 0) ? W : BS;
-#pragma unroll
-            for (int i = 0; i < W; i++) {
-              if (i < ng) {
-                const int row = wb + i;
-                z[i] = (colok && row < n) ? Zc[row] : 0.f;
+            for_chunks<0, W>([&](auto cc) __attribute__((always_inline)) {
+              constexpr int C = decltype(cc)::value;
+              if (C < ng) {
+                stage_load32(xout, Z, ldz, col0, ncols, wb + C, n, lane);
+                from_stage(cc);
               } else {
-                z[i] = xin[(i - BS) * 64 + lane];
+#pragma unroll
+                for (int i = 0; i < 32; i++) z[C + i] = xin[(C - BS + i) * kStageLd + lane];
               }
-            }
+            });
           } else {
 #pragma unroll
             for (int i = 0; i < BS; i++) z[i] = z[i + NB];
+            if (k == 0) {
+              for_chunks<BS, W>([&](auto cc) __attribute__((always_inline)) {
+                stage_load32(xout, Z, ldz, col0, ncols, wb + decltype(cc)::value, n, lane);
+                from_stage(cc);
+              });
+            } else {
 #pragma unroll
-            for (int i = 0; i < NB; i++) {
-              const int row = wb + BS + i;
-              z[BS + i] = (k == 0) ? ((colok && row < n) ? Zc[row] : 0.f) : xin[i * 64 + lane];
+              for (int i = 0; i < NB; i++) z[BS + i] = xin[i * kStageLd + lane];
             }
           }
         } else {
           if (e == 0) {
+            for_chunks<0, W>([&](auto cc) __attribute__((always_inline)) {
+              constexpr int C = decltype(cc)::value;
+              if (k > 0 && C < NB) {
 #pragma unroll
-            for (int i = 0; i < W; i++) {
-              if (k > 0 && i < NB) {
-                z[i] = xin[i * 64 + lane];
+                for (int i = 0; i < 32; i++) z[C + i] = xin[(C + i) * kStageLd + lane];
               } else {
-                const int row = wb + i;
-                z[i] = (colok && row < n) ? Zc[row] : 0.f;
+                stage_load32(xout, Z, ldz, col0, ncols, wb + C, n, lane);
+                from_stage(cc);
               }
-            }
+            });
           } else {
 #pragma unroll
             for (int i = BS - 1; i >= 0; i--) z[NB + i] = z[i];
+            if (k == 0) {
+              for_chunks<0, NB>([&](auto cc) __attribute__((always_inline)) {
+                stage_load32(xout, Z, ldz, col0, ncols, wb + decltype(cc)::value, n, lane);
+                from_stage(cc);
+              });
+            } else {
 #pragma unroll
-            for (int i = 0; i < NB; i++) {
-              const int row = wb + i;
-              z[i] = (k == 0) ? ((colok && row < n) ? Zc[row] : 0.f) : xin[i * 64 + lane];
+              for (int i = 0; i < NB; i++) z[i] = xin[i * kStageLd + lane];
             }
           }
         }
         // ---- apply the BS reflectors of this block at step jj
-        if (S0 + BS <= nsweeps && wb + BS + NB <= n)
-          apply_block_full<NB, BS, W, REV, 0>(z, U, ldu, S0, wb);
-        else
-          apply_block<NB, BS, W, REV, 0>(z, U, ldu, S0, wb, n, nsweeps);
-        // ---- hand on the finished rows
+        if (jj < pack_steps(A, m, bk))
+          apply_packed_block<NB, BS, W, REV>(z, P + (pack_prefix(A, m, bk) + jj) * (long)(BS * NB), lane);
+        // ---- hand on the finished rows (the last wave writes them to Z)
         const bool last = (k == nact - 1);
         if (REV) {
+          for_chunks<0, NB>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int C = decltype(cc)::value;
 #pragma unroll
-          for (int i = 0; i < NB; i++) {
-            const int row = wb + i;
-            if (last) {
-              if (colok && row < n) Zc[row] = z[i];
-            } else {
-              xout[i * 64 + lane] = z[i];
-            }
-          }
+            for (int i = 0; i < 32; i++) xout[(C + i) * kStageLd + lane] = z[C + i];
+            if (last) stage_store32(xout + C * kStageLd, Z, ldz, col0, ncols, wb + C, n, lane);
+          });
         } else {
-#pragma unroll
-          for (int i = 0; i < NB; i++) {
-            const int row = wb + BS + i;
-            if (last) {
-              if (colok && row < n) Zc[row] = z[BS + i];
-            } else {
-              xout[i * 64 + lane] = z[BS + i];
-            }
-          }
           if (e == J - 1) {
             // rows above the next wave's window are final
+            for_chunks<0, BS>([&](auto cc) __attribute__((always_inline)) {
+              constexpr int C = decltype(cc)::value;
 #pragma unroll
-            for (int i = 0; i < BS; i++) {
-              const int row = wb + i;
-              if (colok && row < n) Zc[row] = z[i];
-            }
+              for (int i = 0; i < 32; i++) xout[i * kStageLd + lane] = z[C + i];
+              stage_store32(xout, Z, ldz, col0, ncols, wb + C, n, lane);
+            });
           }
+          for_chunks<0, NB>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int C = decltype(cc)::value;
+#pragma unroll
+            for (int i = 0; i < 32; i++) xout[(C + i) * kStageLd + lane] = z[BS + C + i];
+            if (last) stage_store32(xout + C * kStageLd, Z, ldz, col0, ncols, wb + BS + C, n, lane);
+          });
         }
       }
       __syncthreads();
@@ -196,16 +353,29 @@ __global__ __launch_bounds__(NW * 64) void bcback_kernel(int n, int ncols, float
   }
 }
 
+size_t bc_back_pack_bytes(long n, long b) {
+  if (b == 64) return align256(pack_floats<64, 64>(n) * sizeof(float));
+  return align256(std::max(pack_floats<32, 32>(n), pack_floats<32, 64>(n)) * sizeof(float));
+}
+
 template <int NB, int BS, int NW>
 static int launch_bcback(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev) {
   const int nsweeps = (int)std::max<long>(0, n - 2);
   if (nsweeps == 0 || ncols <= 0) return 0;
+  ArenaFrame frame(ctx);
+  float *P = arena_alloc_n<float>(ctx, pack_floats<NB, BS>(n));
+  if (!P) return EVD_E_NOMEM;
+  const unsigned nblk = (unsigned)((nsweeps + BS - 1) / BS);
+  bc_pack_kernel<NB, BS><<<dim3(64, nblk), 256, 0, ctx->stream>>>((int)n, U, ldu, P, nsweeps);
+  EVD_LAUNCHED(ctx);
+  if (ctx->timing) hipEventRecord(ctx->kev[0], ctx->stream);
   dim3 grid((unsigned)((ncols + 63) / 64));
   if (rev)
-    bcback_kernel<NB, BS, NW, true><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, U, ldu, nsweeps);
+    bcback_kernel<NB, BS, NW, true><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, P, nsweeps);
   else
-    bcback_kernel<NB, BS, NW, false><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, U, ldu, nsweeps);
+    bcback_kernel<NB, BS, NW, false><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, P, nsweeps);
   EVD_LAUNCHED(ctx);
+  if (ctx->timing) hipEventRecord(ctx->kev[1], ctx->stream);
   return 0;
 }
 
@@ -238,14 +408,6 @@ extern "C" int evd_bc_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *
   EVD_ARG(ctx, ldu >= n, 8);
   EVD_ARG(ctx, order == 0 || order == 1, 9);
   if (n <= 2 || ncols == 0) return 0;
-  // the kernel reads up to b rows past the last reflector: run on a copy of
-  // the caller's U whose rows n .. n+b-1 are zero
-  const long ldp = n + b;
-  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)ldp * n * sizeof(float) + 4096));
-  ArenaFrame frame(ctx);
-  float *Up = arena_alloc_n<float>(ctx, (size_t)ldp * n);
-  if (!Up) return EVD_E_NOMEM;
-  EVD_TRY(fill_f32(ctx, Up, ldp, n, ldp, 0.f));
-  EVD_TRY(copy_f32(ctx, U, ldu, Up, ldp, n, n));
-  return bc_back(ctx, n, b, ncols, Q, ldq, Up, ldp, order == 1);
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, bc_back_pack_bytes(n, b) + 4096));
+  return bc_back(ctx, n, b, ncols, Q, ldq, U, ldu, order == 1);
 }

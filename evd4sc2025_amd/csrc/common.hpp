@@ -51,7 +51,8 @@ struct evd_ctx {
 
   bool timing = false;
   hipEvent_t ev[EVD_STAGE_COUNT + 1] = {};
-  float stage_ms[EVD_STAGE_COUNT] = {};
+  hipEvent_t kev[2] = {};  // around the BC back-transformation kernel
+  float stage_ms[EVD_TIMER_COUNT] = {};
 
   char err[1024] = {0};
 };

@@ -181,6 +181,11 @@ int evd_ctx_create(int device, void *stream, evd_ctx **out) {
       delete ctx;
       return EVD_E_HIP;
     }
+  for (auto &e : ctx->kev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return EVD_E_HIP;
+    }
   *out = ctx;
   return 0;
 }
@@ -192,6 +197,8 @@ int evd_ctx_destroy(evd_ctx *ctx) {
   if (ctx->ctrl) hipFree(ctx->ctrl);
   if (ctx->tags) hipFree(ctx->tags);
   for (auto &e : ctx->ev)
+    if (e) hipEventDestroy(e);
+  for (auto &e : ctx->kev)
     if (e) hipEventDestroy(e);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -256,7 +263,7 @@ int evd_ctx_enable_timing(evd_ctx *ctx, int on) {
 int evd_ctx_stage_ms(const evd_ctx *ctx, float *ms, int count) {
   if (!ctx) return -1;
   if (!ms) return -2;
-  for (int i = 0; i < count && i < EVD_STAGE_COUNT; i++) ms[i] = ctx->stage_ms[i];
+  for (int i = 0; i < count && i < EVD_TIMER_COUNT; i++) ms[i] = ctx->stage_ms[i];
   return 0;
 }
 

@@ -23,6 +23,7 @@ int band_pack(evd_ctx *ctx, long n, long b, const float *A, long lda, double *ba
 int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E);
 int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz);
 size_t stedc_workspace(long n);
+size_t bc_back_pack_bytes(long n, long b);
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols);
@@ -123,6 +124,7 @@ size_t syevd_workspace(long n, long b, long nb, char jobz) {
     size_t dc = stedc_workspace(n);
     size_t back = align256((size_t)1024 * n * sizeof(float)) + align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
                   ((size_t)64 << 20);
+    back = std::max(back, bc_back_pack_bytes(n, b) + ((size_t)1 << 20));
     later = std::max(later, std::max(dc, back));
   }
   return s + later + ((size_t)4 << 20);
@@ -182,6 +184,9 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
       hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]);
       ctx->stage_ms[i] = ms;
     }
+    float km = 0.f;
+    if (vec) hipEventElapsedTime(&km, ctx->kev[0], ctx->kev[1]);
+    ctx->stage_ms[EVD_TIMER_BCBACK_KERNEL] = km;
   }
   return 0;
 }

@@ -46,6 +46,10 @@ typedef struct evd_ctx evd_ctx;
 #define EVD_STAGE_BCBACK 3
 #define EVD_STAGE_SBRBACK 4
 #define EVD_STAGE_COUNT 5
+/* extra timer of evd_ctx_stage_ms(): the BC back-transformation kernel alone
+ * (the stage time EVD_STAGE_BCBACK also covers packing its reflectors) */
+#define EVD_TIMER_BCBACK_KERNEL 5
+#define EVD_TIMER_COUNT 6
 
 /* ---- context ---------------------------------------------------------- */
 
@@ -70,7 +74,8 @@ int evd_ctx_set_budget(evd_ctx *ctx, size_t bytes);
 int evd_ctx_set_tuning(evd_ctx *ctx, int bc_sweeps, int bc_waves, int chase_groups, int sbr_back_block);
 int evd_ctx_set_precision(evd_ctx *ctx, int precision);
 /* Per-stage device time (ms) of the last evd_syevd_f32 call, measured with
- * hipEvents on the context stream; enable with evd_ctx_enable_timing. */
+ * hipEvents on the context stream; enable with evd_ctx_enable_timing.  count
+ * may be up to EVD_TIMER_COUNT (ms[5] = the BC back-transformation kernel). */
 int evd_ctx_enable_timing(evd_ctx *ctx, int on);
 int evd_ctx_stage_ms(const evd_ctx *ctx, float *ms, int count);
 int evd_ctx_synchronize(evd_ctx *ctx);
