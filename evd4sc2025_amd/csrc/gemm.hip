@@ -217,13 +217,16 @@ __global__ void splitk_reduce_kernel(int M, int N, long batches, int ksplit, con
 
 template <bool TA, bool TB, int BM, int BN, int WM, int WN>
 static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
-                        const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch) {
+                        const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
+                        long min_chunk = 256) {
   long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
   int ksplit = 1;
   const long target = 2L * ctx->num_cus;
-  if (tiles < target && k >= 1024) {
+  // split K while the tiles alone leave the chip idle, keeping each chunk at
+  // least min_chunk deep (a deterministic second pass sums the partials)
+  if (tiles < target && k >= 2 * min_chunk) {
     long want = (target + tiles - 1) / tiles;
-    long maxs = k / 256;
+    long maxs = k / min_chunk;
     ksplit = (int)std::min<long>(std::min<long>(want, maxs), 64);
     if (ksplit < 1) ksplit = 1;
   }
@@ -253,14 +256,24 @@ static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const
 template <bool TA, bool TB>
 static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
                           const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch) {
-  if (n <= 32 && m > 64)
-    return launch_sgemm<TA, TB, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  const long target = 2L * ctx->num_cus;
+  if (n <= 32 && m > 64) {
+    // tall-skinny: the row tile is chosen for parallelism first (the K
+    // dimension of these products is often short), then K is split
+    const long t256 = (m + 255) / 256 * batch, t128 = (m + 127) / 128 * batch;
+    if (t256 >= target / 2 || k >= 4096)
+      return launch_sgemm<TA, TB, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    if (t128 >= target / 2)
+      return launch_sgemm<TA, TB, 128, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                                                 128);
+    return launch_sgemm<TA, TB, 64, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128);
+  }
   if (m <= 32 && n > 64)
     return launch_sgemm<TA, TB, 32, 256, 1, 4>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   long big_tiles = ((m + 127) / 128) * ((n + 127) / 128) * batch;
-  if (big_tiles >= 2L * ctx->num_cus || (m > 64 && n > 64 && k < 1024))
+  if (big_tiles >= target || (m > 64 && n > 64 && k < 1024))
     return launch_sgemm<TA, TB, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
-  return launch_sgemm<TA, TB, 64, 64, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  return launch_sgemm<TA, TB, 64, 64, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128);
 }
 
 __global__ void scale_kernel(int M, int N, float beta, float *C, long ldc, long sC, long batch) {
