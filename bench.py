@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--nb", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP16/BF16 split) runs")
     ap.add_argument("--cpu-n", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -154,6 +155,34 @@ def main():
 
     print(f"[bench] timed {args.steps} steps: {dt:.3f} s/EVD, stages {stage_ms}, accuracy {acc}", file=sys.stderr,
           flush=True)
+
+    # mixed-precision forms of the same workload (split 16-bit MFMA products in
+    # SBR and the back transformation; FP64 reflectors and tridiagonal stage)
+    mixed = {}
+    if not args.no_mixed:
+        for name, prec in (("fp16", E.PREC_FP16), ("bf16", E.PREC_BF16)):
+            ctx.set_precision(prec)
+            step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            mst = {k: 0.0 for k in E.TIMERS}
+            for _ in range(args.steps):
+                r = step()
+                for k, v in r.stage_ms.items():
+                    mst[k] += v
+            torch.cuda.synchronize()
+            mdt = (time.perf_counter() - t0) / args.steps
+            entry = {"ms_per_step": mdt * 1e3, "tflops": flops / mdt / 1e12, "speedup_vs_fp32": dt / mdt,
+                     "stage_ms": {k: v / args.steps for k, v in mst.items()}}
+            if not args.no_accuracy:
+                Aw.copy_(A0)
+                r = E.syevd(ctx, Aw, b=b, nb=nb, jobz="V", overwrite=True)
+                entry["accuracy"] = {"backward_error": E.backward_error(ctx, A0, r.w, r.Z),
+                                     "orthogonality": E.checkOrthogonality(ctx, r.Z)}
+            mixed[name] = entry
+            print(f"[bench] mixed {name}: {mdt:.3f} s/EVD ({dt / mdt:.2f}x), {entry.get('accuracy')}", file=sys.stderr,
+                  flush=True)
+        ctx.set_precision(E.PREC_FP32)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     line = {
         "metric": "EVD s & TFLOP/s at n=16384 FP32 b=32; mixed-size solves/s + p99 latency",
@@ -177,6 +206,7 @@ def main():
         "chase_roofline": chase,
         "cpu_baseline": cpu,
         "accuracy": acc,
+        "mixed_precision": mixed or None,
     }
     print(json.dumps(line))
 

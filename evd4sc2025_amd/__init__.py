@@ -25,6 +25,9 @@ __all__ = [
     "STAGES",
 ]
 
+# GEMM precision of SBR and the back transformation (evd_ctx_set_precision)
+PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
+
 STAGES = ("sbr", "bc", "dc", "bc_back", "sbr_back")
 # extra timer after the stages: the BC back-transformation kernel alone
 TIMERS = STAGES + ("bc_back_kernel",)

@@ -7,6 +7,9 @@
 //  * syr2k_lower_mirror: C = alpha (A B^T + B A^T) + beta C on lower tiles
 //    only, the result mirrored to the upper triangle in the epilogue
 //    (the role of tc_ozimmu_syr2k + CpyMatrixL2U in the reference).
+//  * split 16-bit variants of sgemm / syr2k for the mixed-precision mode
+//    (evd_ctx_set_precision): FP32 operands split into FP16 hi+lo or BF16
+//    hi+mid+lo terms on the way into LDS, FP32 accumulation.
 //  * dgemm / dgemm_grouped: FP64 on v_mfma_f64_16x16x4f64 for the
 //    divide-and-conquer merges and the FP64 metrics.
 //
@@ -286,6 +289,10 @@ __global__ void scale_kernel(int M, int N, float beta, float *C, long ldc, long 
   }
 }
 
+static int sgemm_split(evd_ctx *ctx, bool TA, bool TB, long m, long n, long k, float alpha, const float *A, long lda,
+                       long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
+                       bool *handled);
+
 int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, const float *A, long lda,
           const float *B, long ldb, float beta, float *C, long ldc, long batch, long sA, long sB, long sC) {
   if (m <= 0 || n <= 0 || batch <= 0) return 0;
@@ -298,6 +305,11 @@ int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, c
     return 0;
   }
   bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+  if (ctx->precision != EVD_PREC_FP32) {
+    bool handled = false;
+    const int r = sgemm_split(ctx, TA, TB, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, &handled);
+    if (handled) return r;
+  }
   if (!TA && !TB) return sgemm_dispatch<false, false>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   if (TA && !TB) return sgemm_dispatch<true, false>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   if (!TA && TB) return sgemm_dispatch<false, true>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
@@ -348,12 +360,321 @@ __global__ __launch_bounds__(256) void syr2k_kernel(int n, int k, float alpha, c
   }
 }
 
+static int syr2k_split(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
+                       float beta, float *C, long ldc);
+
 int syr2k_lower_mirror(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
                        float beta, float *C, long ldc) {
   if (n <= 0) return 0;
+  if (ctx->precision != EVD_PREC_FP32) return syr2k_split(ctx, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   long nt = (n + 127) / 128;
   long tiles = nt * (nt + 1) / 2;
   syr2k_kernel<<<(unsigned)tiles, 256, 0, ctx->stream>>>((int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
+// ------------------------------------------------------- split 16-bit MFMA
+// Mixed-precision products with FP32 operands in HBM: while a tile is staged
+// into LDS every FP32 value is split into NT 16-bit terms, and the MFMA sums
+// the leading cross products in FP32:
+//   FP16:  x = hi + lo            (22 significant bits), hh + hl + lh
+//   BF16:  x = hi + mid + lo      (24 significant bits), hh + hm + mh + hl + lh + mm
+// The neglected products are below 2^-22 (FP16) / 2^-24 (BF16) of |x||y|, so a
+// product keeps about FP32 accuracy while running on the 16-bit matrix cores.
+// LDS holds each term k-contiguous per row ([row][16 k], padded to 20), so a
+// lane reads its 4 consecutive k of one row with a single 8-byte load, the
+// operand layout of v_mfma_f32_16x16x16{f16,bf16}.
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+constexpr int kLDK = 20;
+
+struct SplitF16 {
+  static constexpr int NT = 2, NP = 3;
+  typedef _Float16 elem;
+  typedef f16x4 vec;
+  __device__ static inline void split(float x, elem (&t)[NT]) {
+    t[0] = (_Float16)x;
+    t[1] = (_Float16)(x - (float)t[0]);
+  }
+  // (A term, B term) of the products, smallest first
+  __device__ static constexpr int pa(int p) { return p == 0 ? 1 : 0; }
+  __device__ static constexpr int pb(int p) { return p == 1 ? 1 : 0; }
+  __device__ static inline f32x4 mfma(vec b, vec a, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(b, a, c, 0, 0, 0);
+  }
+};
+
+__device__ inline short f32_to_bf16_rne(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ inline float bf16_to_f32(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+
+struct SplitBF16 {
+  static constexpr int NT = 3, NP = 6;
+  typedef short elem;
+  typedef bf16x4 vec;
+  __device__ static inline void split(float x, elem (&t)[NT]) {
+    t[0] = f32_to_bf16_rne(x);
+    const float r1 = x - bf16_to_f32(t[0]);
+    t[1] = f32_to_bf16_rne(r1);
+    t[2] = f32_to_bf16_rne(r1 - bf16_to_f32(t[1]));
+  }
+  // products (A, B) smallest first: mm, lh, hl, mh, hm, hh
+  __device__ static constexpr int pa(int p) { return p == 0 ? 1 : p == 1 ? 2 : p == 3 ? 1 : 0; }
+  __device__ static constexpr int pb(int p) { return p == 0 ? 1 : p == 2 ? 2 : p == 4 ? 1 : 0; }
+  __device__ static inline f32x4 mfma(vec b, vec a, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b, a, c, 0, 0, 0);
+  }
+};
+
+template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN>
+__device__ inline void store_split(const TileLoader<TA, TB, BM, BN, WM, WN, float> &ld, typename S::elem *As,
+                                   typename S::elem *Bs, int tid) {
+  using L = TileLoader<TA, TB, BM, BN, WM, WN, float>;
+#pragma unroll
+  for (int q = 0; q < L::NA; q++) {
+    const int idx = tid + 256 * q;
+    int i, kk;
+    if (!TA) {
+      i = idx % BM;
+      kk = idx / BM;
+    } else {
+      kk = idx % kBK;
+      i = idx / kBK;
+    }
+    typename S::elem t[S::NT];
+    S::split(ld.ra[q], t);
+#pragma unroll
+    for (int u = 0; u < S::NT; u++) As[u * BM * kLDK + i * kLDK + kk] = t[u];
+  }
+#pragma unroll
+  for (int q = 0; q < L::NB; q++) {
+    const int idx = tid + 256 * q;
+    int j, kk;
+    if (TB) {
+      j = idx % BN;
+      kk = idx / BN;
+    } else {
+      kk = idx % kBK;
+      j = idx / kBK;
+    }
+    typename S::elem t[S::NT];
+    S::split(ld.rb[q], t);
+#pragma unroll
+    for (int u = 0; u < S::NT; u++) Bs[u * BN * kLDK + j * kLDK + kk] = t[u];
+  }
+}
+
+// split-precision counterpart of tile_mainloop (kBK = 16 = one 16x16x16 MFMA)
+template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN>
+__device__ inline void tile_mainloop_split(const float *__restrict__ A, long lda, const float *__restrict__ B, long ldb,
+                                           int M, int N, int m0, int n0, int kb, int ke, typename S::elem *As0,
+                                           typename S::elem *Bs0, f32x4 (&acc)[BM / WM / 16][BN / WN / 16]) {
+  using L = TileLoader<TA, TB, BM, BN, WM, WN, float>;
+  constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
+  constexpr int SA = S::NT * BM * kLDK, SB = S::NT * BN * kLDK;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  L ld;
+  const int nk = ke > kb ? (ke - kb + kBK - 1) / kBK : 0;
+  if (nk > 0) {
+    ld.load(A, lda, B, ldb, M, N, m0, n0, kb, ke, tid);
+    store_split<S>(ld, As0, Bs0, tid);
+  }
+  __syncthreads();
+  for (int t = 0; t < nk; t++) {
+    const int cur = t & 1;
+    if (t + 1 < nk) ld.load(A, lda, B, ldb, M, N, m0, n0, kb + (t + 1) * kBK, ke, tid);
+    const typename S::elem *As = As0 + cur * SA;
+    const typename S::elem *Bs = Bs0 + cur * SB;
+    typename S::vec a[S::NT][MI], b[S::NT][NI];
+    const int ko = 4 * (lane >> 4);
+#pragma unroll
+    for (int u = 0; u < S::NT; u++) {
+#pragma unroll
+      for (int mi = 0; mi < MI; mi++)
+        a[u][mi] = *reinterpret_cast<const typename S::vec *>(
+            &As[u * BM * kLDK + (wm * (BM / WM) + mi * 16 + (lane & 15)) * kLDK + ko]);
+#pragma unroll
+      for (int ni = 0; ni < NI; ni++)
+        b[u][ni] = *reinterpret_cast<const typename S::vec *>(
+            &Bs[u * BN * kLDK + (wn * (BN / WN) + ni * 16 + (lane & 15)) * kLDK + ko]);
+    }
+#pragma unroll
+    for (int p = 0; p < S::NP; p++)
+#pragma unroll
+      for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+        for (int ni = 0; ni < NI; ni++) acc[mi][ni] = S::mfma(b[S::pb(p)][ni], a[S::pa(p)][mi], acc[mi][ni]);
+    if (t + 1 < nk) store_split<S>(ld, As0 + (cur ^ 1) * SA, Bs0 + (cur ^ 1) * SB, tid);
+    __syncthreads();
+  }
+}
+
+template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256) void sgemm_split_kernel(int M, int N, int K, float alpha,
+                                                          const float *__restrict__ A, long lda, long sA,
+                                                          const float *__restrict__ B, long ldb, long sB, float beta,
+                                                          float *__restrict__ C, long ldc, long sC, int kchunk,
+                                                          int ksplit, float *__restrict__ P, long batches) {
+  constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
+  __shared__ typename S::elem As[2 * S::NT * BM * kLDK];
+  __shared__ typename S::elem Bs[2 * S::NT * BN * kLDK];
+  const long bz = blockIdx.z;
+  const long batch = bz / ksplit;
+  const int kz = (int)(bz % ksplit);
+  A += batch * sA;
+  B += batch * sB;
+  C += batch * sC;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kb = kz * kchunk;
+  const int ke = min(K, kb + kchunk);
+  f32x4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+  tile_mainloop_split<S, TA, TB, BM, BN, WM, WN>(A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++) {
+    const int row = m0 + wm * (BM / WM) + mi * 16 + (lane & 15);
+    if (row >= M) continue;
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        if (col >= N) continue;
+        const float v = acc[mi][ni][r];
+        if (ksplit > 1) {
+          P[((long)kz * batches + batch) * (long)M * N + row + (long)col * M] = v;
+        } else {
+          const long o = row + (long)col * ldc;
+          C[o] = (beta == 0.f) ? alpha * v : alpha * v + beta * C[o];
+        }
+      }
+    }
+  }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void syr2k_split_kernel(int n, int k, float alpha, const float *__restrict__ A,
+                                                          long lda, const float *__restrict__ B, long ldb, float beta,
+                                                          float *__restrict__ C, long ldc) {
+  constexpr int BM = 128, BN = 128, WM = 2, WN = 2, MI = 4, NI = 4;
+  __shared__ typename S::elem As[2 * S::NT * BM * kLDK];
+  __shared__ typename S::elem Bs[2 * S::NT * BN * kLDK];
+  const long t = blockIdx.x;
+  long ti = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
+  while (ti * (ti + 1) / 2 > t) ti--;
+  const long tj = t - ti * (ti + 1) / 2;
+  const int m0 = (int)ti * BM, n0 = (int)tj * BN;
+  f32x4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++)
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+  tile_mainloop_split<S, false, true, BM, BN, WM, WN>(A, lda, B, ldb, n, n, m0, n0, 0, k, As, Bs, acc);
+  tile_mainloop_split<S, false, true, BM, BN, WM, WN>(B, ldb, A, lda, n, n, m0, n0, 0, k, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+#pragma unroll
+  for (int mi = 0; mi < MI; mi++) {
+    const int row = m0 + wm * (BM / WM) + mi * 16 + (lane & 15);
+    if (row >= n) continue;
+#pragma unroll
+    for (int ni = 0; ni < NI; ni++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = n0 + wn * (BN / WN) + ni * 16 + 4 * (lane >> 4) + r;
+        if (col > row || col >= n) continue;
+        const long o = row + (long)col * ldc;
+        const float v = (beta == 0.f) ? alpha * acc[mi][ni][r] : alpha * acc[mi][ni][r] + beta * C[o];
+        C[o] = v;
+        if (col != row) C[col + (long)row * ldc] = v;
+      }
+  }
+}
+
+template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN>
+static int launch_sgemm_split(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
+                              const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch) {
+  const long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
+  int ksplit = 1;
+  const long target = 2L * ctx->num_cus;
+  if (tiles < target && k >= 512) {
+    const long want = (target + tiles - 1) / tiles;
+    ksplit = (int)std::min<long>(std::min<long>(want, k / 256), 64);
+    if (ksplit < 1) ksplit = 1;
+  }
+  long kchunk = (k + ksplit - 1) / ksplit;
+  kchunk = (kchunk + kBK - 1) / kBK * kBK;
+  ksplit = (int)((k + kchunk - 1) / kchunk);
+  if (ksplit < 1) ksplit = 1;
+  float *P = nullptr;
+  ArenaFrame frame(ctx);
+  if (ksplit > 1) {
+    P = arena_alloc_n<float>(ctx, (size_t)ksplit * batch * m * n);
+    if (!P) return EVD_E_NOMEM;
+  }
+  dim3 grid((unsigned)((m + BM - 1) / BM), (unsigned)((n + BN - 1) / BN), (unsigned)(batch * ksplit));
+  sgemm_split_kernel<S, TA, TB, BM, BN, WM, WN><<<grid, 256, 0, ctx->stream>>>(
+      (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch);
+  EVD_LAUNCHED(ctx);
+  if (ksplit > 1) {
+    const long total = m * n * batch;
+    const unsigned g = (unsigned)std::min<long>((total + 255) / 256, 4096);
+    splitk_reduce_kernel<<<g, 256, 0, ctx->stream>>>((int)m, (int)n, batch, ksplit, P, alpha, beta, C, ldc, sC);
+    EVD_LAUNCHED(ctx);
+  }
+  return 0;
+}
+
+template <class S>
+static int sgemm_split_t(evd_ctx *ctx, bool TA, bool TB, long m, long n, long k, float alpha, const float *A, long lda,
+                         long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC,
+                         long batch, bool *handled) {
+  *handled = true;
+  // long-K tall-skinny products (A W with K = the trailing size, the long
+  // reductions of the ZY updates)
+  if (n <= 32 && m > 64 && k >= 1024 && !TB) {
+    if (!TA) return launch_sgemm_split<S, false, false, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    return launch_sgemm_split<S, true, false, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  }
+  // large square-ish products (back transformation, WY merges)
+  if (m > 64 && n > 64 && k >= 64) {
+    if (!TA && !TB) return launch_sgemm_split<S, false, false, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    if (TA && !TB) return launch_sgemm_split<S, true, false, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    if (!TA && TB) return launch_sgemm_split<S, false, true, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  }
+  *handled = false;  // short or odd shapes stay on the FP32 kernels
+  return 0;
+}
+
+static int sgemm_split(evd_ctx *ctx, bool TA, bool TB, long m, long n, long k, float alpha, const float *A, long lda,
+                       long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
+                       bool *handled) {
+  if (ctx->precision == EVD_PREC_FP16)
+    return sgemm_split_t<SplitF16>(ctx, TA, TB, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, handled);
+  return sgemm_split_t<SplitBF16>(ctx, TA, TB, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, handled);
+}
+
+static int syr2k_split(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
+                       float beta, float *C, long ldc) {
+  const long nt = (n + 127) / 128;
+  const long tiles = nt * (nt + 1) / 2;
+  if (ctx->precision == EVD_PREC_FP16)
+    syr2k_split_kernel<SplitF16><<<(unsigned)tiles, 256, 0, ctx->stream>>>((int)n, (int)k, alpha, A, lda, B, ldb, beta,
+                                                                          C, ldc);
+  else
+    syr2k_split_kernel<SplitBF16><<<(unsigned)tiles, 256, 0, ctx->stream>>>((int)n, (int)k, alpha, A, lda, B, ldb,
+                                                                           beta, C, ldc);
   EVD_LAUNCHED(ctx);
   return 0;
 }
