@@ -66,6 +66,55 @@ def cpu_baseline(n: int = 4096, seed: int = 0):
     }
 
 
+def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 30):
+    """Serving form on a bounded sample: nreq requests, n drawn from
+    {256: 50%, 1024: 30%, 4096: 15%, 16384: 5%}, 70% with eigenvectors,
+    Gaussian-entry symmetric matrices resident in HBM.  Saturated throughput
+    with one slot (one-at-a-time solves) and with 4 concurrent slots, then the
+    same Poisson arrival schedule (70% of the one-slot capacity) on both."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.choice([256, 1024, 4096, 16384], size=nreq, p=[0.5, 0.3, 0.15, 0.05])]
+    jobz = ["V" if x else "N" for x in (rng.random(nreq) < 0.7)]
+    gctx = E.Context(0)
+    pristine = [E.gen_symmetric(gctx, n, 1000 + i, 1) for i, n in enumerate(sizes)]
+    torch.cuda.synchronize()
+
+    def run(slots, arrivals):
+        work = []
+        for p in pristine:
+            w = E.colmajor(p.shape[0], p.shape[0])
+            w.copy_(p)
+            work.append(w)
+        torch.cuda.synchronize()
+        srv = E.Server(0, slots=slots, hbm_budget=budget)
+        t0 = time.perf_counter()
+        for i in np.argsort(arrivals, kind="stable"):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            n = sizes[i]
+            srv.submit(work[i], b=32, nb=1024 if n >= 4096 else 256, jobz=jobz[i])
+        srv.wait_all()
+        st = srv.stats()
+        srv.close()
+        del work
+        return {k: st[k] for k in ("solves_per_s", "p50_ms", "p99_ms")} | {"arena_gib": st["arena_bytes"] / 2**30}
+
+    zero = np.zeros(nreq)
+    sat1 = run(1, zero)
+    sat4 = run(4, zero)
+    rate = 0.7 * sat1["solves_per_s"]
+    arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    poi1 = run(1, arr)
+    poi4 = run(4, arr)
+    return {"requests": nreq, "mix": "n in {256: 50%, 1024: 30%, 4096: 15%, 16384: 5%}, 70% jobz=V, Gaussian "
+                                     f"entries, seed {seed}, plain FP32, HBM budget {budget >> 30} GiB",
+            "saturated": {"slots1": sat1, "slots4": sat4, "throughput_ratio": sat4["solves_per_s"] / sat1["solves_per_s"]},
+            "poisson": {"rate_per_s": rate, "slots1": poi1, "slots4": poi4}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +126,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP16/BF16 split) runs")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
+    ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--cpu-n", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -183,6 +234,10 @@ def main():
             print(f"[bench] mixed {name}: {mdt:.3f} s/EVD ({dt / mdt:.2f}x), {entry.get('accuracy')}", file=sys.stderr,
                   flush=True)
         ctx.set_precision(E.PREC_FP32)
+    serving = None
+    if not args.no_serve:
+        serving = serving_bench(E, torch, args.serve_requests)
+        print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     line = {
         "metric": "EVD s & TFLOP/s at n=16384 FP32 b=32; mixed-size solves/s + p99 latency",
@@ -207,6 +262,7 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": acc,
         "mixed_precision": mixed or None,
+        "serving": serving,
     }
     print(json.dumps(line))
 

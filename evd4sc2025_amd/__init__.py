@@ -143,6 +143,11 @@ class Context:
     def synchronize(self):
         self.call("evd_ctx_synchronize")
 
+    def poison_arena(self, byte: int = 0xFF):
+        """Fill the workspace arena with `byte` (0xFF = NaN) before a call, to
+        show that no result reads stale scratch."""
+        self.call("evd_ctx_poison_arena", byte)
+
     def arena_bytes(self) -> int:
         return int(self._lib.evd_ctx_arena_bytes(self.h))
 
@@ -361,3 +366,77 @@ def syevd(ctx: Context, A, b: int = 32, nb: int = 1024, jobz: str = "V", overwri
 
 def syevd_workspace(n: int, b: int, nb: int, jobz: str = "V") -> int:
     return int(lib().evd_syevd_workspace_f32(n, b, nb, _ch(jobz)))
+
+
+class Server:
+    """Request loop for uneven-size eigenproblems (evd_server_* in the C ABI):
+    `slots` concurrent contexts with device-resident workspaces, FIFO
+    admission against an HBM budget, per-request latency.  Submitted tensors
+    must stay alive until the request completes (the server keeps them)."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 0):
+        self._lib = lib()
+        h = C.c_void_p()
+        rc = self._lib.evd_server_create(device, slots, hbm_budget, C.byref(h))
+        if rc:
+            raise EvdError("evd_server_create", rc, "")
+        self._h = h
+        self._keep = {}
+
+    def submit(self, A, b: int = 32, nb: int = 1024, jobz: str = "V"):
+        """Queue A (column-major FP32 CUDA tensor, overwritten).  Returns
+        (id, w, Z) with w, Z filled when the request completes."""
+        torch = _torch()
+        n = A.shape[0]
+        pa, lda = _cm(A)
+        w = torch.empty(max(n, 1), dtype=torch.float64, device=A.device)
+        Z = colmajor(n, n, device=A.device) if jobz in ("V", "v") else None
+        # the slots run on their own non-blocking streams: the work that
+        # produced A (on torch's stream) must be complete before they read it
+        torch.cuda.current_stream(A.device).synchronize()
+        pz, ldz = _cm(Z) if Z is not None else (None, max(n, 1))
+        rid = C.c_long()
+        rc = self._lib.evd_server_submit(self._h, n, b, nb, C.c_void_p(pa), lda, C.c_void_p(w.data_ptr()),
+                                         C.c_void_p(pz) if Z is not None else None, ldz, C.byref(rid))
+        if rc:
+            raise EvdError("evd_server_submit", rc, "")
+        self._keep[rid.value] = (A, w, Z)
+        return rid.value, w, Z
+
+    def wait(self, rid: int) -> int:
+        rc = self._lib.evd_server_wait(self._h, rid)
+        if rc:
+            raise EvdError("evd_server_wait", rc, "")
+        self._keep.pop(rid, None)
+        return rc
+
+    def wait_all(self):
+        rc = self._lib.evd_server_wait_all(self._h)
+        if rc:
+            raise EvdError("evd_server_wait_all", rc, "")
+        self._keep.clear()
+
+    def latency_ms(self, rid: int) -> float:
+        out = C.c_double()
+        self._lib.evd_server_request_ms(self._h, rid, C.byref(out))
+        return out.value
+
+    def stats(self) -> dict:
+        n = C.c_long()
+        sps, p50, p99 = C.c_double(), C.c_double(), C.c_double()
+        ab = C.c_size_t()
+        self._lib.evd_server_stats(self._h, C.byref(n), C.byref(sps), C.byref(p50), C.byref(p99), C.byref(ab))
+        return {"completed": n.value, "solves_per_s": sps.value, "p50_ms": p50.value, "p99_ms": p99.value,
+                "arena_bytes": ab.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.evd_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

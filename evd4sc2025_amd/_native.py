@@ -51,6 +51,15 @@ SIGNATURES = {
     "evd_dgemm": (_i, [_vp, _ch, _ch, _l, _l, _l, _d, _vp, _l, _vp, _l, _d, _vp, _l]),
     "evd_gen_symmetric_f32": (_i, [_vp, _l, _u64, _i, _vp, _l]),
     "evd_gen_spectrum_f32": (_i, [_vp, _l, _u64, _vp, _vp, _l]),
+    "evd_ctx_poison_arena": (_i, [_vp, C.c_uint]),
+    "evd_server_create": (_i, [_i, _i, C.c_size_t, C.POINTER(_vp)]),
+    "evd_server_destroy": (_i, [_vp]),
+    "evd_server_submit": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_l)]),
+    "evd_server_wait": (_i, [_vp, _l]),
+    "evd_server_wait_all": (_i, [_vp]),
+    "evd_server_request_ms": (_i, [_vp, _l, C.POINTER(C.c_double)]),
+    "evd_server_stats": (_i, [_vp, C.POINTER(_l), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                              C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
 }
 
 

@@ -267,6 +267,16 @@ int evd_ctx_stage_ms(const evd_ctx *ctx, float *ms, int count) {
   return 0;
 }
 
+int evd_ctx_poison_arena(evd_ctx *ctx, unsigned byte) {
+  if (!ctx) return -1;
+  if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "poison inside a call");
+  if (!ctx->arena || ctx->arena_bytes == 0) return 0;
+  const size_t words = ctx->arena_bytes / sizeof(unsigned);
+  const unsigned b = byte & 0xFFu;
+  EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(ctx->arena), words, b | (b << 8) | (b << 16) | (b << 24)));
+  return 0;
+}
+
 int evd_ctx_synchronize(evd_ctx *ctx) {
   if (!ctx) return -1;
   EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -79,6 +79,10 @@ int evd_ctx_set_precision(evd_ctx *ctx, int precision);
 int evd_ctx_enable_timing(evd_ctx *ctx, int on);
 int evd_ctx_stage_ms(const evd_ctx *ctx, float *ms, int count);
 int evd_ctx_synchronize(evd_ctx *ctx);
+/* Fill the whole workspace arena with a byte pattern (0xFF: NaN in FP32 and
+ * FP64), stream-ordered.  A check that no result depends on stale scratch;
+ * no counterpart in the reference. */
+int evd_ctx_poison_arena(evd_ctx *ctx, unsigned byte);
 
 /* ---- building blocks with reference counterparts ------------------------ */
 
@@ -194,6 +198,34 @@ int evd_gen_symmetric_f32(evd_ctx *ctx, long n, unsigned long long seed, int kin
  * vectors are N(0, I) from the same Philox stream (stream 1); built in FP64
  * on the GPU, rounded to FP32.  lam: device array of n doubles. */
 int evd_gen_spectrum_f32(evd_ctx *ctx, long n, unsigned long long seed, const double *lam, float *A, long lda);
+
+/* ---------------------------------------------------------------- serving
+ * Request loop for eigenproblems of uneven sizes.  No counterpart in the
+ * reference, whose driver solves one matrix per process run
+ * (src/EVD/myEVD.cu:251-706).  `slots` contexts, each with its own stream and
+ * device-resident workspace arena, run admitted requests concurrently; a
+ * request is admitted (FIFO) when a slot is free and the slot arenas, grown to
+ * its workspace, stay within hbm_budget bytes (0 = unlimited).  A, w, Z are
+ * the caller's device buffers (A column-major n x n FP32, lower triangle read
+ * and overwritten; w n FP64; Z n x n FP32 or NULL for eigenvalues only): A must
+ * be complete when submitted (the slots run on their own non-blocking streams,
+ * unordered with the caller's), and all three must stay valid until the
+ * request completes.  Results do not depend on
+ * what else runs: a request returns the same bits alone and in a mix. */
+typedef struct evd_server evd_server;
+int evd_server_create(int device, int slots, size_t hbm_budget, evd_server **out);
+int evd_server_destroy(evd_server *srv); /* finishes queued requests first */
+/* Non-blocking; *id receives the request id.  EVD_E_NOMEM if the request's
+ * workspace alone exceeds the budget. */
+int evd_server_submit(evd_server *srv, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz,
+                      long *id);
+int evd_server_wait(evd_server *srv, long id); /* status of the request */
+int evd_server_wait_all(evd_server *srv);      /* first nonzero status, else 0 */
+int evd_server_request_ms(evd_server *srv, long id, double *latency_ms);
+/* completed requests, solves/s from first submission to last completion,
+ * p50 / p99 latency (submission to completion), bytes held by slot arenas */
+int evd_server_stats(evd_server *srv, long *completed, double *solves_per_s, double *p50_ms, double *p99_ms,
+                     size_t *arena_bytes);
 
 #ifdef __cplusplus
 }

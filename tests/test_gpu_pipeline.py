@@ -107,3 +107,20 @@ def test_syevd_nonfinite_input_is_reported(ctx):
     # the context stays usable
     r = E.syevd(ctx, E.gen_symmetric(ctx, 64, 0, 0), b=32, nb=32)
     assert np.isfinite(to_host(r.w)).all()
+
+
+def test_syevd_independent_of_stale_scratch(ctx):
+    """The solver reads no workspace it has not written: poisoning the arena
+    (NaN bytes, zeros) leaves every output bit unchanged."""
+    import evd4sc2025_amd as E
+
+    n = 777
+    A = O.gen_symmetric(n, 77, 0)
+    ctx.reserve(E.syevd_workspace(n, 32, 256, "V"))
+    r0 = E.syevd(ctx, to_dev(A), b=32, nb=256)
+    w0, Z0 = to_host(r0.w), to_host(r0.Z)
+    for byte in (0xFF, 0x00):
+        ctx.poison_arena(byte)
+        r = E.syevd(ctx, to_dev(A), b=32, nb=256)
+        assert np.array_equal(to_host(r.w), w0)
+        assert np.array_equal(to_host(r.Z), Z0)

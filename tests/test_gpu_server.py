@@ -1,0 +1,84 @@
+"""Serving loop (evd_server_*): continuous admission of uneven-size requests on
+concurrent slots, HBM-budget admission, and results that do not depend on the
+mix they ran in."""
+import numpy as np
+import pytest
+
+from oracle import evd_oracle as O
+from tests.helpers import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+# (n, jobz, seed): ragged sizes, both job kinds
+MIX = [(256, "V", 1), (1024, "V", 2), (300, "N", 3), (2048, "V", 4), (513, "V", 5), (700, "N", 6), (256, "V", 7),
+       (1536, "V", 8)]
+
+
+def _solve_all(srv, reqs):
+    out = []
+    for n, jobz, seed in reqs:
+        A = to_dev(O.gen_symmetric(n, seed, 0))
+        out.append((n, srv.submit(A, b=32, nb=256, jobz=jobz)))
+    srv.wait_all()
+    return [(to_host(w)[:n], None if Z is None else to_host(Z)) for n, (_, w, Z) in out]
+
+
+def test_server_results_identical_alone_and_in_mix():
+    import evd4sc2025_amd as E
+
+    alone = []
+    srv1 = E.Server(0, slots=1)
+    for req in MIX:
+        alone += _solve_all(srv1, [req])
+    srv1.close()
+    srv4 = E.Server(0, slots=4)
+    mixed = _solve_all(srv4, MIX)
+    st = srv4.stats()
+    srv4.close()
+    assert st["completed"] == len(MIX)
+    for (n, jobz, seed), (w1, Z1), (w4, Z4) in zip(MIX, alone, mixed):
+        assert np.array_equal(w1, w4), (n, seed)
+        if jobz == "V":
+            assert np.array_equal(Z1, Z4), (n, seed)
+        # and each result is a correct eigen-decomposition
+        A = O.gen_symmetric(n, seed, 0).astype(np.float64)
+        wr = np.linalg.eigvalsh(A)
+        assert np.abs(np.sort(w4) - wr).max() < 1e-5 * np.abs(wr).max()
+
+
+def test_server_budget_admission():
+    import evd4sc2025_amd as E
+
+    ws = E.syevd_workspace(1024, 32, 256, "V")
+    # a request whose workspace alone exceeds the budget is refused up front
+    small = E.Server(0, slots=2, hbm_budget=ws // 2)
+    with pytest.raises(E.EvdError):
+        small.submit(to_dev(O.gen_symmetric(1024, 9, 0)), b=32, nb=256)
+    small.close()
+    # a budget for one request at a time still completes every request, and
+    # the slot arenas never exceed it
+    srv = E.Server(0, slots=4, hbm_budget=int(ws * 1.5))
+    res = _solve_all(srv, [(1024, "V", 10), (1024, "V", 11), (512, "V", 12), (1024, "N", 13)])
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 4
+    assert st["arena_bytes"] <= int(ws * 1.5)
+    for (w, Z), (n, seed) in zip(res, [(1024, 10), (1024, 11), (512, 12), (1024, 13)]):
+        wr = np.linalg.eigvalsh(O.gen_symmetric(n, seed, 0).astype(np.float64))
+        assert np.abs(np.sort(w) - wr).max() < 1e-5 * np.abs(wr).max()
+
+
+def test_server_latency_statistics():
+    import evd4sc2025_amd as E
+
+    srv = E.Server(0, slots=3)
+    ids = []
+    for n, jobz, seed in MIX[:5]:
+        rid, _, _ = srv.submit(to_dev(O.gen_symmetric(n, seed, 0)), b=32, nb=256, jobz=jobz)
+        ids.append(rid)
+    srv.wait_all()
+    st = srv.stats()
+    lat = [srv.latency_ms(i) for i in ids]
+    srv.close()
+    assert st["completed"] == 5 and st["solves_per_s"] > 0
+    assert min(lat) > 0 and st["p50_ms"] <= st["p99_ms"] <= max(lat) + 1e-9
