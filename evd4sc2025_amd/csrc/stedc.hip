@@ -19,7 +19,8 @@
 // tree height run in the same launches (grid.y = merge), every merge working
 // in the index range [start, start+N) of n-long scratch arrays and in the
 // diagonal block of the n x n Q / scratch matrices.  The deflation scan is
-// sequential per merge (one thread), everything else is parallel.
+// parallel except for the chain of Givens rotations (processed 64 pairs at a
+// time by one wave).
 #include <algorithm>
 #include <cfloat>
 
@@ -104,10 +105,50 @@ __global__ void dc_sort_kernel(const MergeDesc *__restrict__ md, const double *_
 }
 
 // One thread per merge: deflation scan in sorted order (dlaed2 logic).
-__global__ void dc_deflate_kernel(const MergeDesc *__restrict__ md, MergeOut *__restrict__ mo, DCScratch sc) {
-  if (threadIdx.x != 0) return;
+// Block-wide helpers for one 256-thread workgroup.
+__device__ inline double block_max256(double v, double *red) {
+  v = fmax(v, __shfl_xor(v, 32, 64));
+  for (int o = 16; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+// exclusive prefix of per-thread counts (thread order); returns the total
+__device__ inline int block_scan256(int v, int *sh, int &excl) {
+  __syncthreads();
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < 256; t++) {
+      const int x = sh[t];
+      sh[t] = acc;
+      acc += x;
+    }
+    sh[256] = acc;
+  }
+  __syncthreads();
+  excl = sh[threadIdx.x];
+  return sh[256];
+}
+
+// Deflation of one merge (LAPACK dlaed2), one 256-thread workgroup per merge.
+// z-deflation (rho |z_j| <= tol) is decided in parallel and compacted in
+// order.  The Givens deflation scan over the remaining entries is sequential
+// only through its rotations (pair (p, j) depends on the previous pair only if
+// that pair rotated), so wave 0 tests 64 consecutive pairs at once on the
+// current values, accepts every pair before the first rotation, applies that
+// rotation and continues after it.  The deflated list is the in-order
+// z-deflated run merged with the (few) rotated entries; type counts and the
+// type-stable positions come from block scans.
+__global__ __launch_bounds__(256) void dc_deflate_kernel(const MergeDesc *__restrict__ md, MergeOut *__restrict__ mo,
+                                                         DCScratch sc) {
+  __shared__ double red[4];
+  __shared__ int sh[257];
+  __shared__ int s_nc, s_K, s_nrot;
   const MergeDesc m = md[blockIdx.x];
-  const int N = m.n1 + m.n2, st = m.start;
+  const int N = m.n1 + m.n2, st = m.start, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double rho = 2.0 * m.rho;
   double *ds = sc.ds + st, *zs = sc.zs + st;
   const int *cs = sc.cs + st;
@@ -118,120 +159,210 @@ __global__ void dc_deflate_kernel(const MergeDesc *__restrict__ md, MergeOut *__
   int *df_col = sc.df_col + st;
   int *rot_p = sc.rot_p + st, *rot_n = sc.rot_n + st;
   double *rot_c = sc.rot_c + st, *rot_s = sc.rot_s + st;
-  for (int i = 0; i < N; i++) types[i] = (i < m.n1) ? 1 : 3;
+  int *cand = sc.orig + st;     // candidate list (orig is written later by the secular solve)
+  double *tmp_d = sc.tau + st;  // merge buffer (tau likewise)
+  int *tmp_c = sc.dmap + st;    // (dmap is written after the deflation)
+  // chunk of N owned by this thread, in order
+  const int per = (N + 255) / 256, i0 = min(N, tid * per), i1 = min(N, i0 + per);
+
   double dmax = 0.0, zmax = 0.0;
-  for (int i = 0; i < N; i++) {
+  for (int i = i0; i < i1; i++) {
+    types[i] = (i < m.n1) ? 1 : 3;
     dmax = fmax(dmax, fabs(ds[i]));
     zmax = fmax(zmax, fabs(zs[i]));
   }
+  dmax = block_max256(dmax, red);
+  zmax = block_max256(zmax, red);
   const double eps = DBL_EPSILON * 0.5;
   const double tol = 8.0 * eps * fmax(dmax, zmax);
-  int K = 0, nd = 0, nrot = 0;
-  if (rho * zmax <= tol) {
-    for (int j = 0; j < N; j++) {
-      df_d[nd] = ds[j];
-      df_col[nd] = cs[j];
-      nd++;
+  const bool all_defl = (rho * zmax <= tol);
+
+  // z-deflated run (in order) and the candidate list, by compaction
+  int nz_loc = 0;
+  for (int i = i0; i < i1; i++) nz_loc += (all_defl || rho * fabs(zs[i]) <= tol) ? 1 : 0;
+  int zoff;
+  const int nzd = block_scan256(nz_loc, sh, zoff);
+  int coff = (i0 - zoff > 0) ? i0 - zoff : 0;  // candidates before my chunk = i0 - (z-deflated before it)
+  coff = i0 - zoff;
+  for (int i = i0; i < i1; i++) {
+    if (all_defl || rho * fabs(zs[i]) <= tol) {
+      df_d[zoff] = ds[i];
+      df_col[zoff] = cs[i];
+      zoff++;
+    } else {
+      cand[coff++] = i;
     }
-  } else {
-    int pj = -1;
-    for (int j = 0; j < N; j++) {
-      if (rho * fabs(zs[j]) <= tol) {
-        df_d[nd] = ds[j];
-        df_col[nd] = cs[j];
-        nd++;
-        continue;
+  }
+  if (tid == 0) s_nc = N - nzd;
+  __syncthreads();
+  const int nc = s_nc;
+
+  // Givens scan (wave 0)
+  if (wave == 0) {
+    int K = 0, nrot = 0;
+    int pj = nc > 0 ? cand[0] : -1;
+    int k = 1;
+    while (k < nc) {
+      const int nvalid = min(64, nc - k);
+      const bool valid = lane < nvalid;
+      const int j = valid ? cand[k + lane] : 0;
+      const int prv = (lane == 0) ? pj : (valid ? cand[k + lane - 1] : 0);
+      bool rot = false;
+      double c = 0.0, sn = 0.0, tau = 0.0;
+      if (valid) {
+        sn = zs[prv];
+        c = zs[j];
+        tau = hypot(c, sn);
+        const double t = ds[j] - ds[prv];
+        c /= tau;
+        sn = -sn / tau;
+        rot = fabs(t * c * sn) <= tol;
       }
-      if (pj < 0) {
-        pj = j;
-        continue;
+      const unsigned long long mask = __ballot(rot);
+      const int first = mask ? __ffsll((long long)mask) - 1 : nvalid;
+      if (lane < first) {
+        nd_d[K + lane] = ds[prv];
+        nd_z[K + lane] = zs[prv];
+        nd_col[K + lane] = cs[prv];
       }
-      double s = zs[pj], c = zs[j];
-      const double tau = hypot(c, s);
-      const double t = ds[j] - ds[pj];
-      c /= tau;
-      s = -s / tau;
-      if (fabs(t * c * s) <= tol) {
-        zs[j] = tau;
-        zs[pj] = 0.0;
-        const int cp = cs[pj], cn = cs[j];
-        if (types[cp] != types[cn]) types[cn] = 2;
-        types[cp] = 4;
-        rot_p[nrot] = cp;
-        rot_n[nrot] = cn;
-        rot_c[nrot] = c;
-        rot_s[nrot] = s;
+      K += first;
+      if (first < nvalid) {
+        if (lane == first) {
+          zs[j] = tau;
+          zs[prv] = 0.0;
+          const int cp = cs[prv], cn = cs[j];
+          if (types[cp] != types[cn]) types[cn] = 2;
+          types[cp] = 4;
+          rot_p[nrot] = cp;
+          rot_n[nrot] = cn;
+          rot_c[nrot] = c;
+          rot_s[nrot] = sn;
+          const double tt = ds[prv] * c * c + ds[j] * sn * sn;
+          ds[j] = ds[prv] * sn * sn + ds[j] * c * c;
+          ds[prv] = tt;
+          df_d[nzd + nrot] = tt;
+          df_col[nzd + nrot] = cp;
+        }
         nrot++;
-        const double tt = ds[pj] * c * c + ds[j] * s * s;
-        ds[j] = ds[pj] * s * s + ds[j] * c * c;
-        ds[pj] = tt;
-        df_d[nd] = tt;
-        df_col[nd] = cp;
-        nd++;
-        pj = j;
+        pj = __shfl(j, first, 64);
+        k += first + 1;
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       } else {
-        nd_d[K] = ds[pj];
-        nd_z[K] = zs[pj];
-        nd_col[K] = cs[pj];
-        K++;
-        pj = j;
+        pj = __shfl(j, nvalid - 1, 64);
+        k += nvalid;
       }
     }
     if (pj >= 0) {
-      nd_d[K] = ds[pj];
-      nd_z[K] = zs[pj];
-      nd_col[K] = cs[pj];
+      if (lane == 0) {
+        nd_d[K] = ds[pj];
+        nd_z[K] = zs[pj];
+        nd_col[K] = cs[pj];
+      }
       K++;
     }
-  }
-  // deflated list ascending (insertion sort; nearly sorted already)
-  for (int i = 1; i < nd; i++) {
-    double v = df_d[i];
-    int c = df_col[i];
-    int j = i - 1;
-    while (j >= 0 && df_d[j] > v) {
-      df_d[j + 1] = df_d[j];
-      df_col[j + 1] = df_col[j];
-      j--;
+    if (lane == 0) {
+      s_K = K;
+      s_nrot = nrot;
     }
-    df_d[j + 1] = v;
-    df_col[j + 1] = c;
   }
-  int K1 = 0, K2 = 0, K3 = 0;
-  double zsum = 0.0;
-  for (int k = 0; k < K; k++) {
-    int ty = types[nd_col[k]];
-    K1 += (ty == 1);
-    K2 += (ty == 2);
-    K3 += (ty == 3);
-    zsum += nd_z[k] * nd_z[k];
+  __syncthreads();
+  const int K = s_K, nrot = s_nrot, nd = nzd + nrot;
+
+  // deflated list ascending: the rotated entries (few) sorted in place, then
+  // merged with the z-deflated run (already ascending; ties keep the run first)
+  if (tid == 0) {
+    for (int i = nzd + 1; i < nd; i++) {
+      const double v = df_d[i];
+      const int cc = df_col[i];
+      int j = i - 1;
+      while (j >= nzd && df_d[j] > v) {
+        df_d[j + 1] = df_d[j];
+        df_col[j + 1] = df_col[j];
+        j--;
+      }
+      df_d[j + 1] = v;
+      df_col[j + 1] = cc;
+    }
   }
-  int p1 = 0, p2 = K1, p3 = K1 + K2;
-  for (int k = 0; k < K; k++) {
-    int ty = types[nd_col[k]];
-    qpos[k] = (ty == 1) ? p1++ : (ty == 2) ? p2++ : p3++;
+  __syncthreads();
+  if (nrot > 0) {
+    for (int e = tid; e < nd; e += 256) {
+      const double v = df_d[e];
+      int pos;
+      if (e < nzd) {  // rank among rotated entries with value < v
+        int lo = nzd, hi = nd;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (df_d[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        pos = e + (lo - nzd);
+      } else {  // rank among z-deflated entries with value <= v
+        int lo = 0, hi = nzd;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (df_d[mid] <= v) lo = mid + 1; else hi = mid;
+        }
+        pos = (e - nzd) + lo;
+      }
+      tmp_d[pos] = v;
+      tmp_c[pos] = df_col[e];
+    }
+    __syncthreads();
+    for (int e = tid; e < nd; e += 256) {
+      df_d[e] = tmp_d[e];
+      df_col[e] = tmp_c[e];
+    }
   }
-  MergeOut o;
-  o.K = K;
-  o.K1 = K1;
-  o.K2 = K2;
-  o.K3 = K3;
-  o.ndefl = nd;
-  o.nrot = nrot;
-  o.rho2 = rho;
-  o.zsum = zsum;
-  mo[blockIdx.x] = o;
+
+  // type counts, z sum and type-stable positions of the non-deflated entries
+  const int kp = (K + 255) / 256, k0 = min(K, tid * kp), k1 = min(K, k0 + kp);
+  int c1 = 0, c2 = 0, c3 = 0;
+  double zs2 = 0.0;
+  for (int kk = k0; kk < k1; kk++) {
+    const int ty = types[nd_col[kk]];
+    c1 += (ty == 1);
+    c2 += (ty == 2);
+    c3 += (ty == 3);
+    zs2 += nd_z[kk] * nd_z[kk];
+  }
+  int e1, e2, e3;
+  const int K1 = block_scan256(c1, sh, e1);
+  const int K2 = block_scan256(c2, sh, e2);
+  const int K3 = block_scan256(c3, sh, e3);
+  for (int kk = k0; kk < k1; kk++) {
+    const int ty = types[nd_col[kk]];
+    qpos[kk] = (ty == 1) ? e1++ : (ty == 2) ? K1 + e2++ : K1 + K2 + e3++;
+  }
+  // z sum in a fixed order (deterministic): per-thread chunks, then thread order
+  __syncthreads();
+  __shared__ double zpart[256];
+  zpart[tid] = zs2;
+  __syncthreads();
+  if (tid == 0) {
+    double zsum = 0.0;
+    for (int t = 0; t < 256; t++) zsum += zpart[t];
+    MergeOut o;
+    o.K = K;
+    o.K1 = K1;
+    o.K2 = K2;
+    o.K3 = K3;
+    o.ndefl = nd;
+    o.nrot = nrot;
+    o.rho2 = rho;
+    o.zsum = zsum;
+    mo[blockIdx.x] = o;
+  }
 }
 
-// Row-parallel: apply the deflation rotations to Q's columns and gather the
-// permuted copy Qc = [non-deflated by type | deflated].
-__global__ void dc_rotate_gather_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo,
-                                        double *__restrict__ Z, long ldz, DCScratch sc, long ldq) {
+// Row-parallel: apply the deflation rotations to Q's columns.
+__global__ void dc_rotate_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo,
+                                 double *__restrict__ Z, long ldz, DCScratch sc) {
   const MergeDesc m = md[blockIdx.y];
   const MergeOut o = mo[blockIdx.y];
+  if (o.nrot == 0) return;
   const int N = m.n1 + m.n2, st = m.start;
   double *Zb = Z + st + (long)st * ldz;
-  double *Qc = sc.Qc + st + (long)st * ldq;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
     for (int t = 0; t < o.nrot; t++) {
       const int p = sc.rot_p[st + t], q = sc.rot_n[st + t];
@@ -240,18 +371,40 @@ __global__ void dc_rotate_gather_kernel(const MergeDesc *__restrict__ md, const 
       Zb[r + (long)p * ldz] = c * x + s * y;
       Zb[r + (long)q * ldz] = c * y - s * x;
     }
-    for (int k = 0; k < o.K; k++) Qc[r + (long)sc.qpos[st + k] * ldq] = Zb[r + (long)sc.nd_col[st + k] * ldz];
-    for (int t = 0; t < o.ndefl; t++) Qc[r + (long)(o.K + t) * ldq] = Zb[r + (long)sc.df_col[st + t] * ldz];
+  }
+}
+
+// Gather the permuted copy Qc = [non-deflated by type | deflated]: blocks of
+// 256 rows x 64 destination columns (grid z over merges).
+__global__ void dc_gather_kernel(const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo,
+                                 const double *__restrict__ Z, long ldz, DCScratch sc, long ldq) {
+  const MergeDesc m = md[blockIdx.z];
+  const MergeOut o = mo[blockIdx.z];
+  const int N = m.n1 + m.n2, st = m.start;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c0 = blockIdx.y * 64, c1 = min(N, c0 + 64);
+  if (r >= N || c0 >= N) return;
+  const double *Zb = Z + st + (long)st * ldz;
+  double *Qc = sc.Qc + st + (long)st * ldq;
+  for (int k = c0; k < c1; k++) {
+    if (k < o.K)
+      Qc[r + (long)sc.qpos[st + k] * ldq] = Zb[r + (long)sc.nd_col[st + k] * ldz];
+    else
+      Qc[r + (long)k * ldq] = Zb[r + (long)sc.df_col[st + (k - o.K)] * ldz];
   }
 }
 
 // f(tau) pieces relative to pole `org`: psi over poles <= lower, phi over the rest.
+// The secular function and its split derivatives at d[org] + tau, summed by
+// one wave: lane l takes poles l, l+64, ...; a butterfly allreduce leaves the
+// same sums in every lane, so the root iteration stays wave-uniform.
 __device__ inline void secular_eval(const double *__restrict__ d, const double *__restrict__ z, int K, int org,
                                     int lower, double tau, double rho, double &f, double &psi, double &dpsi,
                                     double &phi, double &dphi) {
   const double dorg = d[org];
   psi = dpsi = phi = dphi = 0.0;
-  for (int k = 0; k < K; k++) {
+  const int lane = threadIdx.x & 63;
+  for (int k = lane; k < K; k += 64) {
     const double del = (d[k] - dorg) - tau;
     const double zk = z[k];
     const double t = zk / del;
@@ -265,6 +418,10 @@ __device__ inline void secular_eval(const double *__restrict__ d, const double *
       dphi += dterm;
     }
   }
+  psi = wave_sum(psi);
+  dpsi = wave_sum(dpsi);
+  phi = wave_sum(phi);
+  dphi = wave_sum(dphi);
   psi *= rho;
   dpsi *= rho;
   phi *= rho;
@@ -280,7 +437,9 @@ __global__ void dc_secular_kernel(const MergeDesc *__restrict__ md, const MergeO
   const double *d = sc.nd_d + st, *z = sc.nd_z + st;
   const double rho = o.rho2;
   const double eps = DBL_EPSILON * 0.5;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K; i += gridDim.x * blockDim.x) {
+  // one wave per root
+  const int waves = blockDim.x >> 6, lane = threadIdx.x & 63;
+  for (int i = blockIdx.x * waves + (threadIdx.x >> 6); i < K; i += gridDim.x * waves) {
     int org;
     double lo, hi;
     if (i < K - 1) {
@@ -370,10 +529,12 @@ __global__ void dc_secular_kernel(const MergeDesc *__restrict__ md, const MergeO
       }
       tau = tn;
     }
-    if (!conv) atomicOr(err, (unsigned)EVD_E_NOCONVERGE);
-    sc.orig[st + i] = org;
-    sc.tau[st + i] = tau;
-    sc.lam[st + i] = d[org] + tau;
+    if (lane == 0) {
+      if (!conv) atomicOr(err, (unsigned)EVD_E_NOCONVERGE);
+      sc.orig[st + i] = org;
+      sc.tau[st + i] = tau;
+      sc.lam[st + i] = d[org] + tau;
+    }
   }
 }
 
@@ -583,7 +744,7 @@ int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
     const unsigned gx = (unsigned)std::min(64, (maxN + 255) / 256);
     dc_sort_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, D, Z, ldz, sc);
     EVD_LAUNCHED(ctx);
-    dc_deflate_kernel<<<nm, 64, 0, ctx->stream>>>(lmd, lmo, sc);
+    dc_deflate_kernel<<<nm, 256, 0, ctx->stream>>>(lmd, lmo, sc);
     EVD_LAUNCHED(ctx);
     EVD_HIP(ctx, hipMemcpyAsync(hmo.data() + off, lmo, sizeof(MergeOut) * nm, hipMemcpyDeviceToHost, ctx->stream));
     EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -593,11 +754,15 @@ int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
       maxK = std::max(maxK, hmo[off + i].K);
       maxDefl = std::max<long>(maxDefl, (long)hmo[off + i].ndefl * (all[off + i].n1 + all[off + i].n2));
     }
-    dc_rotate_gather_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, lmo, Z, ldz, sc, ldq);
+    dc_rotate_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, lmo, Z, ldz, sc);
+    EVD_LAUNCHED(ctx);
+    dc_gather_kernel<<<dim3((unsigned)((maxN + 255) / 256), (unsigned)((maxN + 63) / 64), nm), 256, 0, ctx->stream>>>(
+        lmd, lmo, Z, ldz, sc, ldq);
     EVD_LAUNCHED(ctx);
     if (maxK > 0) {
       const unsigned gk = (unsigned)std::min(256, (maxK + 255) / 256);
-      dc_secular_kernel<<<dim3(gk, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, err);
+      const unsigned gw = (unsigned)std::min(16384, (maxK + 3) / 4);  // 4 roots (waves) per workgroup
+      dc_secular_kernel<<<dim3(gw, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, err);
       EVD_LAUNCHED(ctx);
       dc_loewner_kernel<<<dim3(gk, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc);
       EVD_LAUNCHED(ctx);
