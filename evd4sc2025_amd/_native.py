@@ -45,6 +45,7 @@ SIGNATURES = {
     "evd_sbr_back_f32": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
     "evd_syevd_f32": (_i, [_vp, _ch, _l, _l, _l, _vp, _l, _vp, _vp, _l]),
     "evd_syevd_workspace_f32": (_sz, [_l, _l, _l, _ch]),
+    "evd_syevd_workspace_ex": (_sz, [_l, _l, _l, _ch, _i]),
     "evd_orthogonality_f32": (_i, [_vp, _l, _l, _vp, _l, C.POINTER(_d)]),
     "evd_backward_error_f32": (_i, [_vp, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_d)]),
     "evd_sgemm": (_i, [_vp, _ch, _ch, _l, _l, _l, _f, _vp, _l, _vp, _l, _f, _vp, _l]),

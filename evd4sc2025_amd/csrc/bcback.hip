@@ -379,7 +379,22 @@ static int launch_bcback(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, c
   return 0;
 }
 
+size_t bc_back_mixed_bytes(long n);
+bool bc_back_mixed_fits(long n, long ldq);
+int bc_back_mixed(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu);
+
+// The FP16 mixed-precision form of Q <- H_1 .. H_K Q runs as WY blocks on the
+// matrix cores (bcback_mfma.hip); everything else on the FP32 VALU kernel.
+static bool use_mixed(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
+  return ctx->precision == EVD_PREC_FP16 && rev && b == 32 && bc_back_mixed_fits(n, ldq);
+}
+
+size_t bc_back_bytes(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
+  return use_mixed(ctx, n, b, ldq, rev) ? bc_back_mixed_bytes(n) : bc_back_pack_bytes(n, b);
+}
+
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev) {
+  if (use_mixed(ctx, n, b, ldq, rev)) return bc_back_mixed(ctx, n, ncols, Q, ldq, U, ldu);
   const int bs = ctx->tune.bc_sweeps;
   const int nw = ctx->tune.bc_waves;
   if (b == 32) {
@@ -408,6 +423,6 @@ extern "C" int evd_bc_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *
   EVD_ARG(ctx, ldu >= n, 8);
   EVD_ARG(ctx, order == 0 || order == 1, 9);
   if (n <= 2 || ncols == 0) return 0;
-  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, bc_back_pack_bytes(n, b) + 4096));
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, bc_back_bytes(ctx, n, b, ldq, order == 1) + 4096));
   return bc_back(ctx, n, b, ncols, Q, ldq, U, ldu, order == 1);
 }

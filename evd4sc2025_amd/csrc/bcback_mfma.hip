@@ -1,0 +1,429 @@
+// Mixed-precision back transformation of the bulge-chasing reflectors on the
+// 16-bit matrix cores: Q <- H_1 H_2 .. H_K Q (the product the eigenvector
+// path needs, the role of my_BC_back_trans_v8_10_noBandU,
+// api/BC_backTransV8_10_noBandU.cu:60-313), with the reflectors aggregated
+// into compact WY blocks and applied as split-FP16 MFMA products.
+//
+// Blocks.  Sweeps are cut into blocks of 32; block (S, j) holds the 32
+// reflectors of sweeps 32S-1 .. 32S+30 at step j (sweep -1 does not exist and
+// counts as the identity).  Reflector t of the block (I - u u^T, |u|^2 = 2)
+// covers rows r0 + t .. r0 + t + 31 with r0 = 32S + 32j, so the block is a
+// 64 x 32 staircase V (column t nonzero in window rows t .. t+31; row 63 is
+// always zero) on a window that starts on a 128-byte line of Q's columns.  The block's product in
+// sweep order is H_0 H_1 .. H_31 = I - V T V^T with T upper triangular (the
+// forward columnwise recurrence T[0:t, t] = -T[0:t, 0:t] V[:, 0:t]^T v_t,
+// T[t][t] = 1), and applying it is
+//     X = V^T Zw   (32 x c),     Zw <- Zw - (V T) X   (64 x c)
+// on the 64-row window Zw of the block.  Blocks run S descending, j ascending
+// (reflectors that share rows are applied higher sweep first).
+//
+// Precision.  V T and V are formed in FP64 (bcw_prep_kernel) and stored as
+// FP16 hi + lo pairs; the window is kept in FP32 accumulators scaled by 2^10
+// and split into hi + lo right before it feeds an MFMA, X likewise.  Each
+// product sums lo*hi + hi*lo + hi*hi in FP32 (22 significant bits per
+// operand).  The scale keeps the lo terms of eigenvector entries out of the
+// FP16 subnormal range; it requires columns of Q with 2-norm <= 8 (the
+// eigenvector matrix of the tridiagonal stage has unit columns), so that
+// |Zw| and |X| stay below the FP16 range.
+//
+// MI355X mapping.  One workgroup owns 64 columns of Q; its NW waves form a
+// pipeline over NW consecutive sweep blocks (wave k works on block S_top - k
+// one step behind wave k-1) and hand each other the 32 rows they have
+// finished through an LDS double buffer, so Q is streamed from HBM once per
+// 32 NW sweeps.  A wave holds its 64 x 64 window as 4 x 4 accumulator tiles of
+// v_mfma_f32_16x16x32_f16; the K index of the 32-deep operands is permuted so
+// that the two 16-row accumulator tiles a lane holds are exactly the 8 K
+// values its operand slot needs (kmap below): accumulators become operands
+// without any data movement between lanes.  The two window halves alternate
+// roles (top/bottom) from step to step instead of being copied.
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace evd {
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kB = 32;                 // reflector length = sweeps per block
+constexpr int kFrag = 64;              // one operand fragment: 64 lanes x 8 halves
+constexpr int kBlockFrags = 16;        // V^T (4 fragments) and -V T (4), hi and lo each
+constexpr float kScale = 1024.f;       // window scale (2^10)
+constexpr float kUnscale = 1.f / 1024.f;
+constexpr float kHalfMax = 60000.f;    // range guard for the stored operands
+
+// K slot e (0..7) of lane group g of a 32-deep operand <-> row of the two
+// 16-row accumulator tiles the lane holds (rows 4g..4g+3 and 16+4g..16+4g+3).
+__host__ __device__ constexpr int kmap(int g, int e) { return e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4); }
+
+// sweep block S has Jn - S steps (Jn = ceil(n / 32): windows start below n),
+// and its blocks start at index prefix(S)
+__host__ __device__ inline long wy_steps(long Jn, long S) { return Jn - S; }
+__host__ __device__ inline long wy_prefix(long Jn, long S) { return S * Jn - S * (S - 1) / 2; }
+__host__ __device__ inline long wy_sweep_blocks(long nsweeps) { return (nsweeps + kB) / kB; }
+
+// ------------------------------------------------------------------ prep
+// One workgroup per block (S, j): V from U, T by the forward recurrence, V T,
+// and the MFMA operand fragments of V^T and -(V T), hi and lo halves.
+// Fragment layout per block (in units of 64-lane h8 fragments):
+//   [0..7]  V^T: fragment f = 2 ks + mt (ks: 32-row K step, mt: 16-sweep tile), term u at 2 f + u
+//   [8..15] -V T: fragment 4 + mt (mt: 16-row tile), term u at 8 + 2 mt + u
+__global__ __launch_bounds__(256) void bcw_prep_kernel(int n, const float *__restrict__ U, long ldu, int nsweeps,
+                                                       h8 *__restrict__ F, unsigned *__restrict__ err) {
+  const long Jn = (n + kB - 1) / kB;
+  const int S = blockIdx.y, j = blockIdx.x;
+  if (j >= wy_steps(Jn, S)) return;
+  __shared__ double V[64][33];
+  __shared__ double G[32][33];
+  __shared__ double T[32][33];
+  __shared__ double VT[64][33];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < 64 * 32; idx += 256) {
+    const int r = idx & 63, t = idx >> 6, i = r - t;
+    const long sw = (long)S * kB - 1 + t;
+    const long row = (long)S * kB + (long)j * kB + r;
+    double v = 0.0;
+    if (i >= 0 && i < kB && sw >= 0 && sw < nsweeps && row < n) v = (double)U[row + sw * ldu];
+    V[r][t] = v;
+  }
+  for (int idx = tid; idx < 32 * 32; idx += 256) T[idx & 31][idx >> 5] = ((idx & 31) == (idx >> 5)) ? 1.0 : 0.0;
+  __syncthreads();
+  // G[a][c] = v_a . v_c for a < c: the columns overlap in rows c .. a+31
+  for (int idx = tid; idx < 32 * 32; idx += 256) {
+    const int a = idx & 31, c = idx >> 5;
+    double s = 0.0;
+    if (a < c)
+      for (int r = c; r < a + kB; r++) s += V[r][a] * V[r][c];
+    G[a][c] = s;
+  }
+  __syncthreads();
+  for (int t = 1; t < kB; t++) {
+    if (tid < t) {
+      double s = 0.0;
+      for (int q = tid; q < t; q++) s += T[tid][q] * G[q][t];
+      T[tid][t] = -s;
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < 64 * 32; idx += 256) {
+    const int r = idx & 63, t = idx >> 6;
+    double s = 0.0;
+    for (int q = 0; q <= t; q++) s += V[r][q] * T[q][t];
+    VT[r][t] = s;
+  }
+  __syncthreads();
+  const long blk = wy_prefix(Jn, S) + j;
+  h8 *out = F + blk * (kBlockFrags * kFrag);
+  bool bad = false;
+  for (int idx = tid; idx < 8 * kFrag; idx += 256) {
+    const int f = idx >> 6, lane = idx & 63, g = lane >> 4, m = lane & 15;
+    h8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      double x;
+      if (f < 4) {
+        const int ks = f >> 1, mt = f & 1;
+        x = V[32 * ks + kmap(g, e)][16 * mt + m];
+      } else {
+        x = -VT[16 * (f - 4) + m][kmap(g, e)];
+      }
+      const float xf = (float)x;
+      bad |= !(fabsf(xf) < kHalfMax);
+      const _Float16 h = (_Float16)xf;
+      hi[e] = h;
+      lo[e] = (_Float16)(float)(x - (double)(float)h);
+    }
+    out[(2 * f) * kFrag + lane] = hi;
+    out[(2 * f + 1) * kFrag + lane] = lo;
+  }
+  if (bad) atomicOr(err, (unsigned)EVD_E_RANGE);
+}
+
+// ------------------------------------------------------------------ main
+// x - float(h) for the low / high FP16 half of h, one v_fma_mix each
+__device__ __attribute__((always_inline)) inline float resid_lo(h2 h, float x) {
+  float r;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+  return r;
+}
+__device__ __attribute__((always_inline)) inline float resid_hi(h2 h, float x) {
+  float r;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+  return r;
+}
+
+// split 8 FP32 values (4 from each of two accumulator tiles) into FP16 hi, lo
+__device__ __attribute__((always_inline)) inline void split8(const f4 &a, const f4 &b, h8 &hi, h8 &lo) {
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    const float x0 = e < 4 ? a[e] : b[e - 4], x1 = e < 4 ? a[e + 1] : b[e - 3];
+    const h2 h = __builtin_convertvector(f2{x0, x1}, h2);
+    const h2 l = __builtin_convertvector(f2{resid_lo(h, x0), resid_hi(h, x1)}, h2);
+    hi[e] = h[0];
+    hi[e + 1] = h[1];
+    lo[e] = l[0];
+    lo[e + 1] = l[1];
+  }
+}
+
+__device__ __attribute__((always_inline)) inline f4 mma(const h8 &a, const h8 &b, const f4 &c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// Element (row r, column c) of a 32 x 64 half window in accumulator order
+// (tile 4 mt + nt, lane 16 g + c16, component i), as a float index.
+__device__ __attribute__((always_inline)) inline int half_index(int r, int c) {
+  return (((r >> 4) * 4 + (c >> 4)) * 64 + ((r >> 2) & 3) * 16 + (c & 15)) * 4 + (r & 3);
+}
+
+// LDS slot: 32 rows x 64 columns in accumulator order
+__device__ __attribute__((always_inline)) inline void put_half(f4 *slot, const f4 (&h)[2][4], int lane) {
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) slot[(mt * 4 + nt) * 64 + lane] = h[mt][nt];
+}
+__device__ __attribute__((always_inline)) inline void get_half(f4 (&h)[2][4], const f4 *slot, int lane) {
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) h[mt][nt] = slot[(mt * 4 + nt) * 64 + lane];
+}
+
+// fragments of a block: V^T of K step ks (4 x h8: mt 0 hi, lo, mt 1 hi, lo), -V T (8 x h8)
+template <int N>
+__device__ __attribute__((always_inline)) inline void load_frags(h8 (&fr)[N], const h8 *__restrict__ src, int lane) {
+#pragma unroll
+  for (int q = 0; q < N; q++) fr[q] = src[q * kFrag + lane];
+}
+
+// Zw <- (I - V T V^T) Zw for one block; top = window rows 0..31, bot = 32..63.
+// v0 / v1 hold V^T for window rows 0..31 / 32..63, wt holds -V T.  With nxt
+// given, the next block's v0 and wt are loaded as soon as the current ones are
+// consumed, so their latency hides behind the rest of the step.
+__device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4], f4 (&bot)[2][4], h8 (&v0)[4],
+                                                                  const h8 (&v1)[4], h8 (&wt)[8],
+                                                                  const h8 *__restrict__ nxt, int lane) {
+  f4 x[2][4];
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) x[mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
+  // X = V^T Zw: K = 64 window rows in two steps, one column tile at a time
+  // (products lo*hi, hi*lo, hi*hi)
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    h8 zh, zl;
+    split8(top[0][nt], top[1][nt], zh, zl);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) x[mt][nt] = mma(v0[2 * mt + 1], zh, x[mt][nt]);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) x[mt][nt] = mma(v0[2 * mt], zl, x[mt][nt]);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) x[mt][nt] = mma(v0[2 * mt], zh, x[mt][nt]);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (nxt) load_frags(v0, nxt, lane);
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    h8 zh, zl;
+    split8(bot[0][nt], bot[1][nt], zh, zl);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) x[mt][nt] = mma(v1[2 * mt + 1], zh, x[mt][nt]);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) x[mt][nt] = mma(v1[2 * mt], zl, x[mt][nt]);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) x[mt][nt] = mma(v1[2 * mt], zh, x[mt][nt]);
+  }
+  // Zw += (-V T) X, K = 32 sweeps
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    h8 xh, xl;
+    split8(x[0][nt], x[1][nt], xh, xl);
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+      const int ua = u == 0 ? 1 : 0;  // lo*hi, hi*lo, hi*hi
+      const h8 &b = (u == 1) ? xl : xh;
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        if (mt < 2)
+          top[mt][nt] = mma(wt[2 * mt + ua], b, top[mt][nt]);
+        else
+          bot[mt - 2][nt] = mma(wt[2 * mt + ua], b, bot[mt - 2][nt]);
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (nxt) load_frags(wt, nxt + 8 * kFrag, lane);
+}
+
+__device__ __attribute__((always_inline)) inline void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// HBM <-> LDS traffic of a half window, spread over the 8 waves: wave k moves
+// columns 8k .. 8k+7, one instruction = 2 columns x 32 consecutive rows.
+// The accesses are raw buffer operations over the workgroup's 64 columns that
+// are always issued: a row or column that must be skipped gets an offset past
+// the buffer, which the hardware drops (stores) or reads as 0 (loads).  With a
+// fixed number of memory operations per step the compiler can wait for the
+// block fragments without also waiting for this traffic.
+constexpr unsigned kBufWord3 = 0x00020000u;  // raw buffer descriptor, 32-bit data
+constexpr unsigned kOutside = 0x80000000u;   // offset beyond every buffer here
+struct HalfIO {
+  __amdgpu_buffer_rsrc_t rs;
+  int r, c;         // row of the lane, first column of the lane (within the 64)
+  unsigned cbytes;  // byte offset of the lane's first column
+  int ncl;          // columns of this workgroup
+  __device__ HalfIO(float *Zc, long ldz, int n, int ncols_wg, int lane, int k)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(Zc, 0, (int)(((long)(ncols_wg - 1) * ldz + n) * 4), kBufWord3)),
+        r(lane & 31),
+        c(8 * k + (lane >> 5)),
+        cbytes((unsigned)(c * ldz * 4)),
+        ncl(ncols_wg) {}
+  __device__ __attribute__((always_inline)) inline unsigned off(bool on, long row0, int n, long ldz, int q) const {
+    const long gr = row0 + r;
+    return (on && gr < n && c + 2 * q < ncl) ? cbytes + (unsigned)((2 * q * ldz + gr) * 4) : kOutside;
+  }
+  __device__ __attribute__((always_inline)) inline void load(float (&v)[4], bool on, long row0, int n, long ldz) const {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off(on, row0, n, ldz, q), 0, 0)) *
+             kScale;
+  }
+  __device__ __attribute__((always_inline)) inline void park(f4 *slot, const float (&v)[4]) const {
+    float *s = reinterpret_cast<float *>(slot);
+#pragma unroll
+    for (int q = 0; q < 4; q++) s[half_index(r, c + 2 * q)] = v[q];
+  }
+  __device__ __attribute__((always_inline)) inline void store(const f4 *slot, bool on, long row0, int n,
+                                                              long ldz) const {
+    const float *s = reinterpret_cast<const float *>(slot);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float v = s[half_index(r, c + 2 * q)] * kUnscale;
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off(on, row0, n, ldz, q), 0, 0);
+    }
+  }
+};
+
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *__restrict__ Z, long ldz,
+                                                      const h8 *__restrict__ F, int nsweeps) {
+  static_assert(NW == 8, "each wave moves 8 of the 64 columns of a half window");
+  // hand-off slots (2 per wave), the rows wave 0 takes in at each step, and
+  // the top rows of each wave's first step (both double-buffered): 160 KiB
+  __shared__ f4 xch[NW * 2 * 512];
+  __shared__ f4 stage[2 * 512];
+  __shared__ f4 stage_top[2 * 512];
+  const int lane = threadIdx.x & 63;
+  const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col0 = blockIdx.x * 64;
+  const int nblk = (int)wy_sweep_blocks(nsweeps);
+  const long Jn = (n + kB - 1) / kB;
+  const HalfIO io(Z + (long)col0 * ldz, ldz, n, min(64, ncols - col0), lane, k);
+  f4 zt[2][4], zb[2][4];
+  h8 v0[4], v1[4], wt[8];
+
+  const int ngroups = (nblk + NW - 1) / NW;
+  for (int gi = 0; gi < ngroups; gi++) {
+    const int gtop = nblk - 1 - gi * NW;
+    const int nact = min(NW, gtop + 1);
+    const int bk = gtop - k;
+    // steps of the lowest block, plus one that hands its last rows on
+    const int J = (int)wy_steps(Jn, gtop - nact + 1) + 1;
+    const int T = J + nact - 1;
+    const int kl = nact - 1;  // last active wave
+    const long nst = wy_steps(Jn, bk);
+    const h8 *fb = F + wy_prefix(Jn, bk) * (long)(kBlockFrags * kFrag);
+    // One pipeline step.  Rows are loaded two steps before their use into
+    // (lb, lt) and parked in LDS one step later from (pb, pt); the loop is
+    // unrolled by two so the two register sets alternate without copies
+    // (a copy would wait for the load).
+    auto body = [&](int tau, float(&lb)[4], float(&lt)[4], const float(&pb)[4], const float(&pt)[4])
+                    __attribute__((always_inline)) {
+      const int e = tau - k;
+      const bool act = tau >= 0 && tau < T && k < nact && e >= 0 && e < J;
+      const bool app = act && e < nst;
+      const h8 *cur = fb + (long)e * (kBlockFrags * kFrag);
+      // (0) this step's fragments first: waiting for them then never waits
+      // for the HBM traffic issued below
+      if (app) {
+        if (e == 0) {
+          load_frags(v0, cur, lane);
+          load_frags(wt, cur + 8 * kFrag, lane);
+        }
+        load_frags(v1, cur + 4 * kFrag, lane);
+      }
+      // (1) rows the last wave handed on at step tau-1 -> Q
+      const int el = tau - 1 - kl;
+      io.store(xch + (kl * 2 + (el & 1)) * 512, el >= 0 && el < J, (long)(gtop - kl + el) * kB, n, ldz);
+      // (2) rows needed at step tau+2: wave 0's new bottom half, and the top
+      // half of wave tau+2's first step
+      io.load(lb, tau + 2 < J, (long)(gtop + tau + 3) * kB, n, ldz);
+      io.load(lt, tau + 2 < nact, (long)(gtop - tau - 2) * kB, n, ldz);
+      // (3) this wave's step
+      if (act) {
+        if (e == 0) {
+          get_half(zt, stage_top + (k & 1) * 512, lane);
+        } else {
+#pragma unroll
+          for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) zt[mt][nt] = zb[mt][nt];
+        }
+        get_half(zb, k == 0 ? stage + (e & 1) * 512 : xch + ((k - 1) * 2 + (e & 1)) * 512, lane);
+        if (app) apply_block(zt, zb, v0, v1, wt, (e + 1 < nst && e + 1 < J) ? cur + kBlockFrags * kFrag : nullptr, lane);
+        put_half(xch + (k * 2 + (e & 1)) * 512, zt, lane);
+      }
+      // (4) park the rows loaded at step tau-1 (for step tau+1)
+      if (tau >= -1 && tau + 1 < J) io.park(stage + ((tau + 1) & 1) * 512, pb);
+      if (tau >= -1 && tau + 1 < nact) io.park(stage_top + ((tau + 1) & 1) * 512, pt);
+      lds_barrier();
+    };
+    float la[4], ta[4], lb2[4], tb2[4];
+    for (int tau = -2; tau <= T; tau += 2) {
+      body(tau, la, ta, lb2, tb2);
+      if (tau + 1 <= T) body(tau + 1, lb2, tb2, la, ta);
+    }
+    // the next group reads rows stored above
+    wait_mem();
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+size_t bc_back_mixed_bytes(long n) {
+  if (n <= 2) return 0;
+  return align256((size_t)wy_prefix((n + kB - 1) / kB, wy_sweep_blocks(n - 2)) * kBlockFrags * kFrag * sizeof(h8));
+}
+
+// the 64 columns of a workgroup must be addressable with 32-bit buffer offsets
+bool bc_back_mixed_fits(long n, long ldq) { return (63 * ldq + n) * 4 < (1L << 31); }
+
+// Q <- H_1 .. H_K Q for b = 32 on split-FP16 MFMA (columns of Q with 2-norm <= 8)
+int bc_back_mixed(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu) {
+  const int nsweeps = (int)std::max<long>(0, n - 2);
+  if (nsweeps == 0 || ncols <= 0) return 0;
+  ArenaFrame frame(ctx);
+  h8 *F = static_cast<h8 *>(arena_alloc(ctx, bc_back_mixed_bytes(n)));
+  if (!F) return EVD_E_NOMEM;
+  const long Jn = (n + kB - 1) / kB, nblk = wy_sweep_blocks(nsweeps);
+  bcw_prep_kernel<<<dim3((unsigned)Jn, (unsigned)nblk), 256, 0, ctx->stream>>>(
+      (int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError);
+  EVD_LAUNCHED(ctx);
+  if (ctx->timing) hipEventRecord(ctx->kev[0], ctx->stream);
+  constexpr int NW = 8;
+  bcw_kernel<NW><<<dim3((unsigned)((ncols + 63) / 64)), NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F,
+                                                                                    nsweeps);
+  EVD_LAUNCHED(ctx);
+  if (ctx->timing) hipEventRecord(ctx->kev[1], ctx->stream);
+  return 0;
+}
+
+}  // namespace evd

@@ -24,7 +24,7 @@
 #include "common.hpp"
 
 namespace evd {
-size_t syevd_workspace(long n, long b, long nb, char jobz);
+size_t syevd_workspace(long n, long b, long nb, char jobz, int precision);
 }
 
 namespace {
@@ -164,7 +164,7 @@ int evd_server_submit(evd_server *s, long n, long b, long nb, float *A, long lda
   if (!w && n > 0) return -7;
   if (Z && ldz < std::max<long>(1, n)) return -9;
   if (!id) return -10;
-  const size_t ws = evd::syevd_workspace(n, b, nb, Z ? 'V' : 'N');
+  const size_t ws = evd::syevd_workspace(n, b, nb, Z ? 'V' : 'N', EVD_PREC_FP32);
   if (ws > s->budget) return EVD_E_NOMEM;
   {
     std::lock_guard<std::mutex> lk(s->mu);

@@ -24,6 +24,7 @@ int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, lon
 int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz);
 size_t stedc_workspace(long n);
 size_t bc_back_pack_bytes(long n, long b);
+size_t bc_back_mixed_bytes(long n);
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols);
@@ -111,7 +112,7 @@ __global__ void copy_f64_kernel(long n, const double *__restrict__ s, double *__
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
-size_t syevd_workspace(long n, long b, long nb, char jobz) {
+size_t syevd_workspace(long n, long b, long nb, char jobz, int precision) {
   const long wc = wy_cols(n);
   size_t s = 0;
   s += align256((size_t)n * wc * sizeof(float)) * 2;          // W, Y
@@ -125,6 +126,7 @@ size_t syevd_workspace(long n, long b, long nb, char jobz) {
     size_t back = align256((size_t)1024 * n * sizeof(float)) + align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
                   ((size_t)64 << 20);
     back = std::max(back, bc_back_pack_bytes(n, b) + ((size_t)1 << 20));
+    if (precision == EVD_PREC_FP16 && b == 32) back = std::max(back, bc_back_mixed_bytes(n) + ((size_t)1 << 20));
     later = std::max(later, std::max(dc, back));
   }
   return s + later + ((size_t)4 << 20);
@@ -232,7 +234,12 @@ using namespace evd;
 
 extern "C" size_t evd_syevd_workspace_f32(long n, long b, long nb, char jobz) {
   if (n <= 0) return 0;
-  return syevd_workspace(n, b, nb, jobz);
+  return syevd_workspace(n, b, nb, jobz, EVD_PREC_FP32);
+}
+
+extern "C" size_t evd_syevd_workspace_ex(long n, long b, long nb, char jobz, int precision) {
+  if (n <= 0) return 0;
+  return syevd_workspace(n, b, nb, jobz, precision);
 }
 
 extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z,
@@ -249,7 +256,7 @@ extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, f
   EVD_ARG(ctx, !vec || Z != nullptr || n == 0, 9);
   EVD_ARG(ctx, !vec || ldz >= std::max<long>(1, n), 10);
   if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_f32 cannot nest");
-  EVD_TRY(arena_ensure(ctx, syevd_workspace(n, b, nb, jobz)));
+  EVD_TRY(arena_ensure(ctx, syevd_workspace(n, b, nb, jobz, ctx->precision)));
   return syevd(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
 }
 

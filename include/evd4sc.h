@@ -33,6 +33,7 @@ typedef struct evd_ctx evd_ctx;
 #define EVD_E_TIMEOUT 4    /* a persistent kernel's bounded wait expired */
 #define EVD_E_UNSUPPORTED 5
 #define EVD_E_NOCONVERGE 6 /* secular equation did not converge */
+#define EVD_E_RANGE 7      /* a mixed-precision operand left the 16-bit range */
 
 /* precision of the GEMM work in SBR and the back transformation */
 #define EVD_PREC_FP32 0 /* plain FP32 MFMA / FP32 VALU */
@@ -168,8 +169,11 @@ int evd_sbr_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ld
  * destroyed).  b = bandwidth (32 or 64), nb = SBR block.  Replaces main's
  * SBR -> BC -> D&C -> back-transformation sequence (src/EVD/myEVD.cu:251-571). */
 int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz);
-/* Arena bytes evd_syevd_f32 needs for (n, b, nb, jobz). */
+/* Arena bytes evd_syevd_f32 needs for (n, b, nb, jobz) in plain FP32. */
 size_t evd_syevd_workspace_f32(long n, long b, long nb, char jobz);
+/* The same for a given EVD_PREC_* (the FP16 form's back transformation keeps
+ * its WY blocks, 16 KiB per 32 x 32 reflector block, in the arena). */
+size_t evd_syevd_workspace_ex(long n, long b, long nb, char jobz, int precision);
 
 /* ---- metrics (util/tool/checkMetric.cu:5-25, src/EVD/myEVD.cu:598-639) ---- */
 
