@@ -127,6 +127,10 @@ class Context:
     def set_tuning(self, bc_sweeps=0, bc_waves=0, chase_groups=0, sbr_back_block=0):
         self.call("evd_ctx_set_tuning", bc_sweeps, bc_waves, chase_groups, sbr_back_block)
 
+    def set_knob(self, name: str, value: int):
+        """One named tuning knob (evd_ctx_set_knob); 0 restores the default."""
+        self.call("evd_ctx_set_knob", name.encode(), int(value))
+
     def set_precision(self, precision: int):
         self.call("evd_ctx_set_precision", precision)
 

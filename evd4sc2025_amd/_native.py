@@ -28,6 +28,7 @@ SIGNATURES = {
     "evd_ctx_arena_bytes": (_sz, [_vp]),
     "evd_ctx_set_budget": (_i, [_vp, _sz]),
     "evd_ctx_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
+    "evd_ctx_set_knob": (_i, [_vp, C.c_char_p, _i]),
     "evd_ctx_set_precision": (_i, [_vp, _i]),
     "evd_ctx_enable_timing": (_i, [_vp, _i]),
     "evd_ctx_stage_ms": (_i, [_vp, C.POINTER(_f), _i]),

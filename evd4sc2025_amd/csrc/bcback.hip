@@ -379,22 +379,23 @@ static int launch_bcback(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, c
   return 0;
 }
 
-size_t bc_back_mixed_bytes(long n);
-bool bc_back_mixed_fits(long n, long ldq);
-int bc_back_mixed(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu);
+size_t bc_back_wy_bytes(long n);
+bool bc_back_wy_fits(long n, long ldq);
+int bc_back_wy(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu, bool split16);
 
-// The FP16 mixed-precision form of Q <- H_1 .. H_K Q runs as WY blocks on the
-// matrix cores (bcback_mfma.hip); everything else on the FP32 VALU kernel.
-static bool use_mixed(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
-  return ctx->precision == EVD_PREC_FP16 && rev && b == 32 && bc_back_mixed_fits(n, ldq);
+// Q <- H_1 .. H_K Q with b = 32 runs as WY blocks on the matrix cores
+// (bcback_mfma.hip): split FP16 in the FP16 mixed-precision mode, FP32 MFMA
+// otherwise; the other order and b = 64 run on the FP32 VALU kernel below.
+static bool use_wy(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
+  return rev && b == 32 && !ctx->tune.bc_valu && bc_back_wy_fits(n, ldq);
 }
 
 size_t bc_back_bytes(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
-  return use_mixed(ctx, n, b, ldq, rev) ? bc_back_mixed_bytes(n) : bc_back_pack_bytes(n, b);
+  return use_wy(ctx, n, b, ldq, rev) ? bc_back_wy_bytes(n) : bc_back_pack_bytes(n, b);
 }
 
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev) {
-  if (use_mixed(ctx, n, b, ldq, rev)) return bc_back_mixed(ctx, n, ncols, Q, ldq, U, ldu);
+  if (use_wy(ctx, n, b, ldq, rev)) return bc_back_wy(ctx, n, ncols, Q, ldq, U, ldu, ctx->precision == EVD_PREC_FP16);
   const int bs = ctx->tune.bc_sweeps;
   const int nw = ctx->tune.bc_waves;
   if (b == 32) {

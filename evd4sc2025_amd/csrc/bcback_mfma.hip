@@ -53,7 +53,6 @@ constexpr int kB = 32;                 // reflector length = sweeps per block
 constexpr int kFrag = 64;              // one operand fragment: 64 lanes x 8 halves
 constexpr int kBlockFrags = 16;        // V^T (4 fragments) and -V T (4), hi and lo each
 constexpr float kScale = 1024.f;       // window scale (2^10)
-constexpr float kUnscale = 1.f / 1024.f;
 constexpr float kHalfMax = 60000.f;    // range guard for the stored operands
 
 // K slot e (0..7) of lane group g of a 32-deep operand <-> row of the two
@@ -69,9 +68,15 @@ __host__ __device__ inline long wy_sweep_blocks(long nsweeps) { return (nsweeps 
 // ------------------------------------------------------------------ prep
 // One workgroup per block (S, j): V from U, T by the forward recurrence, V T,
 // and the MFMA operand fragments of V^T and -(V T), hi and lo halves.
-// Fragment layout per block (in units of 64-lane h8 fragments):
+// A block is 16 fragments of 64 lanes x 16 bytes.
+// Split FP16 (h8 per lane, K = 32 per MFMA):
 //   [0..7]  V^T: fragment f = 2 ks + mt (ks: 32-row K step, mt: 16-sweep tile), term u at 2 f + u
 //   [8..15] -V T: fragment 4 + mt (mt: 16-row tile), term u at 8 + 2 mt + u
+// FP32 (f4 per lane = the 4 K=4 MFMAs over one 16-row tile, component i
+// taking rows 4g + i of the tile, the rows an accumulator lane holds):
+//   [0..7]  V^T: fragment 2 rt + mt (rt: 16-row window tile, mt: 16-sweep tile)
+//   [8..15] -V T: fragment 8 + 2 mt + st (mt: 16-row window tile, st: 16-sweep tile)
+template <bool F32>
 __global__ __launch_bounds__(256) void bcw_prep_kernel(int n, const float *__restrict__ U, long ldu, int nsweeps,
                                                        h8 *__restrict__ F, unsigned *__restrict__ err) {
   const long Jn = (n + kB - 1) / kB;
@@ -118,6 +123,22 @@ __global__ __launch_bounds__(256) void bcw_prep_kernel(int n, const float *__res
   __syncthreads();
   const long blk = wy_prefix(Jn, S) + j;
   h8 *out = F + blk * (kBlockFrags * kFrag);
+  if constexpr (F32) {
+    f4 *o4 = reinterpret_cast<f4 *>(out);
+    for (int idx = tid; idx < kBlockFrags * kFrag; idx += 256) {
+      const int q = idx >> 6, lane = idx & 63, g = lane >> 4, m = lane & 15;
+      f4 v;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (q < 8)
+          v[i] = (float)V[16 * (q >> 1) + 4 * g + i][16 * (q & 1) + m];
+        else
+          v[i] = (float)-VT[16 * ((q - 8) >> 1) + m][16 * ((q - 8) & 1) + 4 * g + i];
+      }
+      o4[idx] = v;
+    }
+    return;
+  }
   bool bad = false;
   for (int idx = tid; idx < 8 * kFrag; idx += 256) {
     const int f = idx >> 6, lane = idx & 63, g = lane >> 4, m = lane & 15;
@@ -174,6 +195,10 @@ __device__ __attribute__((always_inline)) inline f4 mma(const h8 &a, const h8 &b
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
+__device__ __attribute__((always_inline)) inline f4 mma(float a, float b, const f4 &c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
 // Element (row r, column c) of a 32 x 64 half window in accumulator order
 // (tile 4 mt + nt, lane 16 g + c16, component i), as a float index.
 __device__ __attribute__((always_inline)) inline int half_index(int r, int c) {
@@ -195,8 +220,8 @@ __device__ __attribute__((always_inline)) inline void get_half(f4 (&h)[2][4], co
 }
 
 // fragments of a block: V^T of K step ks (4 x h8: mt 0 hi, lo, mt 1 hi, lo), -V T (8 x h8)
-template <int N>
-__device__ __attribute__((always_inline)) inline void load_frags(h8 (&fr)[N], const h8 *__restrict__ src, int lane) {
+template <typename FR, int N>
+__device__ __attribute__((always_inline)) inline void load_frags(FR (&fr)[N], const FR *__restrict__ src, int lane) {
 #pragma unroll
   for (int q = 0; q < N; q++) fr[q] = src[q * kFrag + lane];
 }
@@ -261,6 +286,58 @@ __device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4
   if (nxt) load_frags(wt, nxt + 8 * kFrag, lane);
 }
 
+// The FP32 form of apply_block: V^T and -V T as f4 fragments, K = 4 per
+// v_mfma_f32_16x16x4f32; accumulator tiles are the B operands as they are.
+__device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4], f4 (&bot)[2][4], f4 (&v0)[4],
+                                                                  const f4 (&v1)[4], f4 (&wt)[8],
+                                                                  const f4 *__restrict__ nxt, int lane) {
+  f4 x[2][4];
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) x[mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
+  // X = V^T Zw.  Sweep tile mt of the staircase only reaches window tiles
+  // mt .. mt+2, and column t of V T only rows 0 .. t+31: the zero tiles
+  // (window tile 0 x sweep tile 1, window tile 3 x sweep tile 0) are skipped.
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+          if (!(rt == 0 && mt == 1)) x[mt][nt] = mma(v0[2 * rt + mt][i], top[rt][nt][i], x[mt][nt]);
+  __builtin_amdgcn_sched_barrier(0);
+  if (nxt) load_frags(v0, nxt, lane);
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+          if (!(rt == 1 && mt == 0)) x[mt][nt] = mma(v1[2 * rt + mt][i], bot[rt][nt][i], x[mt][nt]);
+  // Zw += (-V T) X
+#pragma unroll
+  for (int st = 0; st < 2; st++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+          if (mt == 3 && st == 0) continue;
+          if (mt < 2)
+            top[mt][nt] = mma(wt[2 * mt + st][i], x[st][nt][i], top[mt][nt]);
+          else
+            bot[mt - 2][nt] = mma(wt[2 * mt + st][i], x[st][nt][i], bot[mt - 2][nt]);
+        }
+  __builtin_amdgcn_sched_barrier(0);
+  if (nxt) load_frags(wt, nxt + 8 * kFrag, lane);
+}
+
 __device__ __attribute__((always_inline)) inline void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
@@ -279,12 +356,14 @@ struct HalfIO {
   int r, c;         // row of the lane, first column of the lane (within the 64)
   unsigned cbytes;  // byte offset of the lane's first column
   int ncl;          // columns of this workgroup
-  __device__ HalfIO(float *Zc, long ldz, int n, int ncols_wg, int lane, int k)
+  float scale;      // window scale (loads multiply, stores divide)
+  __device__ HalfIO(float *Zc, long ldz, int n, int ncols_wg, int lane, int k, float sc)
       : rs(__builtin_amdgcn_make_buffer_rsrc(Zc, 0, (int)(((long)(ncols_wg - 1) * ldz + n) * 4), kBufWord3)),
         r(lane & 31),
         c(8 * k + (lane >> 5)),
         cbytes((unsigned)(c * ldz * 4)),
-        ncl(ncols_wg) {}
+        ncl(ncols_wg),
+        scale(sc) {}
   __device__ __attribute__((always_inline)) inline unsigned off(bool on, long row0, int n, long ldz, int q) const {
     const long gr = row0 + r;
     return (on && gr < n && c + 2 * q < ncl) ? cbytes + (unsigned)((2 * q * ldz + gr) * 4) : kOutside;
@@ -292,8 +371,7 @@ struct HalfIO {
   __device__ __attribute__((always_inline)) inline void load(float (&v)[4], bool on, long row0, int n, long ldz) const {
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off(on, row0, n, ldz, q), 0, 0)) *
-             kScale;
+      v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off(on, row0, n, ldz, q), 0, 0)) * scale;
   }
   __device__ __attribute__((always_inline)) inline void park(f4 *slot, const float (&v)[4]) const {
     float *s = reinterpret_cast<float *>(slot);
@@ -305,15 +383,17 @@ struct HalfIO {
     const float *s = reinterpret_cast<const float *>(slot);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const float v = s[half_index(r, c + 2 * q)] * kUnscale;
+      const float v = s[half_index(r, c + 2 * q)] / scale;
       __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off(on, row0, n, ldz, q), 0, 0);
     }
   }
 };
 
-template <int NW>
+template <int NW, bool F32>
 __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *__restrict__ Z, long ldz,
-                                                      const h8 *__restrict__ F, int nsweeps) {
+                                                      const h8 *__restrict__ F16, int nsweeps) {
+  typedef typename std::conditional<F32, f4, h8>::type Frag;
+  const Frag *F = reinterpret_cast<const Frag *>(F16);
   static_assert(NW == 8, "each wave moves 8 of the 64 columns of a half window");
   // hand-off slots (2 per wave), the rows wave 0 takes in at each step, and
   // the top rows of each wave's first step (both double-buffered): 160 KiB
@@ -325,9 +405,9 @@ __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *_
   const int col0 = blockIdx.x * 64;
   const int nblk = (int)wy_sweep_blocks(nsweeps);
   const long Jn = (n + kB - 1) / kB;
-  const HalfIO io(Z + (long)col0 * ldz, ldz, n, min(64, ncols - col0), lane, k);
+  const HalfIO io(Z + (long)col0 * ldz, ldz, n, min(64, ncols - col0), lane, k, F32 ? 1.f : kScale);
   f4 zt[2][4], zb[2][4];
-  h8 v0[4], v1[4], wt[8];
+  Frag v0[4], v1[4], wt[8];
 
   const int ngroups = (nblk + NW - 1) / NW;
   for (int gi = 0; gi < ngroups; gi++) {
@@ -339,7 +419,7 @@ __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *_
     const int T = J + nact - 1;
     const int kl = nact - 1;  // last active wave
     const long nst = wy_steps(Jn, bk);
-    const h8 *fb = F + wy_prefix(Jn, bk) * (long)(kBlockFrags * kFrag);
+    const Frag *fb = F + wy_prefix(Jn, bk) * (long)(kBlockFrags * kFrag);
     // One pipeline step.  Rows are loaded two steps before their use into
     // (lb, lt) and parked in LDS one step later from (pb, pt); the loop is
     // unrolled by two so the two register sets alternate without copies
@@ -349,7 +429,7 @@ __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *_
       const int e = tau - k;
       const bool act = tau >= 0 && tau < T && k < nact && e >= 0 && e < J;
       const bool app = act && e < nst;
-      const h8 *cur = fb + (long)e * (kBlockFrags * kFrag);
+      const Frag *cur = fb + (long)e * (kBlockFrags * kFrag);
       // (0) this step's fragments first: waiting for them then never waits
       // for the HBM traffic issued below
       if (app) {
@@ -398,29 +478,36 @@ __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *_
 
 }  // namespace
 
-size_t bc_back_mixed_bytes(long n) {
+size_t bc_back_wy_bytes(long n) {
   if (n <= 2) return 0;
   return align256((size_t)wy_prefix((n + kB - 1) / kB, wy_sweep_blocks(n - 2)) * kBlockFrags * kFrag * sizeof(h8));
 }
 
 // the 64 columns of a workgroup must be addressable with 32-bit buffer offsets
-bool bc_back_mixed_fits(long n, long ldq) { return (63 * ldq + n) * 4 < (1L << 31); }
+bool bc_back_wy_fits(long n, long ldq) { return (63 * ldq + n) * 4 < (1L << 31); }
 
-// Q <- H_1 .. H_K Q for b = 32 on split-FP16 MFMA (columns of Q with 2-norm <= 8)
-int bc_back_mixed(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu) {
+// Q <- H_1 .. H_K Q for b = 32 as WY blocks on the matrix cores: FP32 MFMA, or
+// split FP16 (split16: columns of Q with 2-norm <= 8)
+int bc_back_wy(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu, bool split16) {
   const int nsweeps = (int)std::max<long>(0, n - 2);
   if (nsweeps == 0 || ncols <= 0) return 0;
   ArenaFrame frame(ctx);
-  h8 *F = static_cast<h8 *>(arena_alloc(ctx, bc_back_mixed_bytes(n)));
+  h8 *F = static_cast<h8 *>(arena_alloc(ctx, bc_back_wy_bytes(n)));
   if (!F) return EVD_E_NOMEM;
   const long Jn = (n + kB - 1) / kB, nblk = wy_sweep_blocks(nsweeps);
-  bcw_prep_kernel<<<dim3((unsigned)Jn, (unsigned)nblk), 256, 0, ctx->stream>>>(
-      (int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError);
+  const dim3 pg((unsigned)Jn, (unsigned)nblk);
+  if (split16)
+    bcw_prep_kernel<false><<<pg, 256, 0, ctx->stream>>>((int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError);
+  else
+    bcw_prep_kernel<true><<<pg, 256, 0, ctx->stream>>>((int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError);
   EVD_LAUNCHED(ctx);
   if (ctx->timing) hipEventRecord(ctx->kev[0], ctx->stream);
   constexpr int NW = 8;
-  bcw_kernel<NW><<<dim3((unsigned)((ncols + 63) / 64)), NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F,
-                                                                                    nsweeps);
+  const dim3 grid((unsigned)((ncols + 63) / 64));
+  if (split16)
+    bcw_kernel<NW, false><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F, nsweeps);
+  else
+    bcw_kernel<NW, true><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F, nsweeps);
   EVD_LAUNCHED(ctx);
   if (ctx->timing) hipEventRecord(ctx->kev[1], ctx->stream);
   return 0;

@@ -23,6 +23,7 @@ struct Tuning {
   int bc_waves = 0;        // pipelined waves per BC-back workgroup (0 = 4)
   int chase_groups = 0;    // persistent workgroups of the bulge chase (0 = auto)
   int sbr_back_block = 0;  // WY block width of the SBR back transformation (0 = 1024)
+  int bc_valu = 0;         // 1: BC back transformation on the FP32 VALU kernel only
 };
 
 }  // namespace evd

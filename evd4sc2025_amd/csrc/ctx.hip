@@ -247,6 +247,25 @@ int evd_ctx_set_tuning(evd_ctx *ctx, int bc_sweeps, int bc_waves, int chase_grou
   return 0;
 }
 
+int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value) {
+  if (!ctx) return -1;
+  if (!name) return -2;
+  struct {
+    const char *name;
+    int *slot;
+    int lo, hi;
+  } knobs[] = {{"bc_sweeps", &ctx->tune.bc_sweeps, 0, 64},          {"bc_waves", &ctx->tune.bc_waves, 0, 16},
+               {"chase_groups", &ctx->tune.chase_groups, 0, 1 << 20}, {"sbr_back_block", &ctx->tune.sbr_back_block, 0, 1 << 16},
+               {"bc_valu", &ctx->tune.bc_valu, 0, 1}};
+  for (auto &kb : knobs)
+    if (!strcmp(kb.name, name)) {
+      if (value < kb.lo || value > kb.hi) return -3;
+      *kb.slot = value;
+      return 0;
+    }
+  return -2;
+}
+
 int evd_ctx_set_precision(evd_ctx *ctx, int precision) {
   if (!ctx) return -1;
   if (precision < EVD_PREC_FP32 || precision > EVD_PREC_FP16) return -2;

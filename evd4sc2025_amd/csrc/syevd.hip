@@ -24,7 +24,7 @@ int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, lon
 int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz);
 size_t stedc_workspace(long n);
 size_t bc_back_pack_bytes(long n, long b);
-size_t bc_back_mixed_bytes(long n);
+size_t bc_back_wy_bytes(long n);
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols);
@@ -126,7 +126,7 @@ size_t syevd_workspace(long n, long b, long nb, char jobz, int precision) {
     size_t back = align256((size_t)1024 * n * sizeof(float)) + align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
                   ((size_t)64 << 20);
     back = std::max(back, bc_back_pack_bytes(n, b) + ((size_t)1 << 20));
-    if (precision == EVD_PREC_FP16 && b == 32) back = std::max(back, bc_back_mixed_bytes(n) + ((size_t)1 << 20));
+    if (b == 32) back = std::max(back, bc_back_wy_bytes(n) + ((size_t)1 << 20));
     later = std::max(later, std::max(dc, back));
   }
   return s + later + ((size_t)4 << 20);

@@ -172,10 +172,15 @@ def test_stedc_matches_lapack(ctx, name, d, e):
     assert O.orthogonality(Z) < 1e-14
 
 
+@pytest.mark.parametrize("valu", [0, 1])
 @pytest.mark.parametrize("n,b,order", [(70, 32, 1), (70, 32, 0), (300, 32, 1), (300, 32, 0), (257, 64, 1),
                                        (257, 64, 0), (1100, 32, 1)])
-def test_bc_back_matches_oracle(ctx, n, b, order):
+def test_bc_back_matches_oracle(ctx, n, b, order, valu):
+    """order 1 with b = 32 runs as WY blocks on FP32 MFMA (valu = 0) or on the
+    FP32 VALU kernel (valu = 1); the other cases always take the VALU kernel."""
     import evd4sc2025_amd as E
+
+    ctx.set_knob("bc_valu", valu)
 
     A = O.gen_symmetric(n, 400 + n).astype(np.float64)
     Ab, _, _ = O.sy2sb(A, b, 2 * b)
@@ -185,6 +190,7 @@ def test_bc_back_matches_oracle(ctx, n, b, order):
     dX = to_dev(X)
     dU = to_dev(U32)
     E.bc_back(ctx, dX, dU, n, b, order)
+    ctx.set_knob("bc_valu", 0)
     want = O.bc_back(X.astype(np.float64), U32.astype(np.float64), n, b, order)
     assert np.abs(to_host(dX) - want).max() < 2e-5 * np.abs(X).max()
 
