@@ -22,7 +22,7 @@ __all__ = [
     "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "hou_tsqr_panel", "tc_ozimmu_syr2k", "band_pack",
     "sb2st", "stedc", "computerWYFromSlide", "computerQFromWY", "my_BC_back_trans_v8_10_noBandU", "bc_back",
     "sbr_back", "checkOrthogonality", "backward_error", "sgemm", "dgemm", "gen_symmetric", "gen_spectrum",
-    "STAGES",
+    "STAGES", "autotune", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
 ]
 
 # GEMM precision of SBR and the back transformation (evd_ctx_set_precision)
@@ -130,6 +130,9 @@ class Context:
     def set_knob(self, name: str, value: int):
         """One named tuning knob (evd_ctx_set_knob); 0 restores the default."""
         self.call("evd_ctx_set_knob", name.encode(), int(value))
+
+    def use_tune_cache(self, on: bool = True):
+        self.call("evd_ctx_use_tune_cache", 1 if on else 0)
 
     def set_precision(self, precision: int):
         self.call("evd_ctx_set_precision", precision)
@@ -370,6 +373,53 @@ def syevd(ctx: Context, A, b: int = 32, nb: int = 1024, jobz: str = "V", overwri
 
 def syevd_workspace(n: int, b: int, nb: int, jobz: str = "V") -> int:
     return int(lib().evd_syevd_workspace_f32(n, b, nb, _ch(jobz)))
+
+
+# ---- autotuning (evd_autotune / evd_tune_cache_* in the C ABI)
+KNOBS = ("nb", "chase_groups", "bc_valu", "sbr_back_block")
+
+
+def autotune(ctx: Context, n: int, b: int = 32, reps: int = 1) -> dict:
+    """Tune the stage knobs for order-n problems at bandwidth b and the
+    context's precision; the result is stored in the process-wide cache and
+    returned as {knob: value}."""
+    out = (C.c_int * 4)()
+    ctx.call("evd_autotune", n, b, reps, out)
+    return dict(zip(KNOBS, list(out)))
+
+
+def tune_cache_put(n: int, b: int, precision: int, knobs: dict):
+    arr = (C.c_int * 4)(*[int(knobs.get(k, 0)) for k in KNOBS])
+    rc = lib().evd_tune_cache_put(n, b, precision, arr)
+    if rc:
+        raise EvdError("evd_tune_cache_put", rc, "invalid entry")
+
+
+def tune_cache_get(n: int, b: int, precision: int):
+    """{knob: value} of the cache entry covering (n, b, precision), or None."""
+    arr = (C.c_int * 4)()
+    rc = lib().evd_tune_cache_get(n, b, precision, arr)
+    if rc == 1:
+        return None
+    if rc:
+        raise EvdError("evd_tune_cache_get", rc, "")
+    return dict(zip(KNOBS, list(arr)))
+
+
+def tune_cache_clear():
+    lib().evd_tune_cache_clear()
+
+
+def tune_cache_save(path: str):
+    rc = lib().evd_tune_cache_save(str(path).encode())
+    if rc:
+        raise EvdError("evd_tune_cache_save", rc, str(path))
+
+
+def tune_cache_load(path: str):
+    rc = lib().evd_tune_cache_load(str(path).encode())
+    if rc:
+        raise EvdError("evd_tune_cache_load", rc, str(path))
 
 
 class Server:

@@ -47,6 +47,13 @@ SIGNATURES = {
     "evd_syevd_f32": (_i, [_vp, _ch, _l, _l, _l, _vp, _l, _vp, _vp, _l]),
     "evd_syevd_workspace_f32": (_sz, [_l, _l, _l, _ch]),
     "evd_syevd_workspace_ex": (_sz, [_l, _l, _l, _ch, _i]),
+    "evd_autotune": (_i, [_vp, _l, _l, _i, C.POINTER(_i)]),
+    "evd_tune_cache_put": (_i, [_l, _l, _i, C.POINTER(_i)]),
+    "evd_tune_cache_get": (_i, [_l, _l, _i, C.POINTER(_i)]),
+    "evd_tune_cache_clear": (_i, []),
+    "evd_tune_cache_save": (_i, [C.c_char_p]),
+    "evd_tune_cache_load": (_i, [C.c_char_p]),
+    "evd_ctx_use_tune_cache": (_i, [_vp, _i]),
     "evd_orthogonality_f32": (_i, [_vp, _l, _l, _vp, _l, C.POINTER(_d)]),
     "evd_backward_error_f32": (_i, [_vp, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_d)]),
     "evd_sgemm": (_i, [_vp, _ch, _ch, _l, _l, _l, _f, _vp, _l, _vp, _l, _f, _vp, _l]),
@@ -78,6 +85,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is missing: run `python -m evd4sc2025_amd.build` (hipcc, gfx950) first")
+        # The library links libamdhip64 by soname.  torch ships its own HIP
+        # runtime; loading this library first would bind the process to the
+        # system runtime and leave torch on a mismatched one (kernel lookups
+        # then fail with hipErrorUnknown).  Importing torch first makes both
+        # use torch's runtime.
+        import torch  # noqa: F401
+
         h = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_GLOBAL", 0))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(h, name)

@@ -251,9 +251,9 @@ static int launch_chase(evd_ctx *ctx, long n, long b, double *band, long ldb, fl
   if (bytes >= (1L << 31))
     return set_error(ctx, EVD_E_UNSUPPORTED, "band of %ld bytes exceeds the 2 GiB buffer range of the chase", bytes);
   // workgroups: by default as many as are resident (more only queue)
-  int per_cu = 0;
-  EVD_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)chase_kernel<BW>, 256, 0));
-  long G = ctx->tune.chase_groups > 0 ? ctx->tune.chase_groups : (long)std::max(1, per_cu) * ctx->num_cus;
+  int resident = 0;
+  EVD_TRY(max_coresident_blocks(ctx, (const void *)chase_kernel<BW>, 256, 0, &resident));
+  long G = ctx->tune.chase_groups > 0 ? ctx->tune.chase_groups : (long)std::max(ctx->num_cus, resident);
   G = std::min<long>(G, std::max<long>(1, n - 2));
   int ni = (int)n, bi = (int)b;
   unsigned nbytes = (unsigned)bytes;

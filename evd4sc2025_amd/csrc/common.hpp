@@ -48,6 +48,7 @@ struct evd_ctx {
   unsigned epoch = 0;        // launch counter for tag-based exchanges
   unsigned *tags = nullptr;  // kTagWords persistent exchange tags, zeroed at creation
   evd::Tuning tune;
+  bool tune_cache = true;  // fill knobs left at 0 from the process-wide tuning cache
   int precision = EVD_PREC_FP32;
 
   bool timing = false;

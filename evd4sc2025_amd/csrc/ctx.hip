@@ -1,4 +1,8 @@
 // Context, arena and small stream-ordered utility kernels.
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "common.hpp"
 
 namespace evd {
@@ -113,8 +117,21 @@ int copy_f32(evd_ctx *ctx, const float *src, long lds, float *dst, long ldd, lon
 }
 
 int max_coresident_blocks(evd_ctx *ctx, const void *kernel, int block, size_t dyn_smem, int *out) {
-  int per_cu = 0;
-  EVD_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn_smem));
+  // per (kernel, block, LDS, device): a property of the code object
+  static std::mutex mu;
+  static std::map<std::tuple<const void *, int, size_t, int>, int> known;
+  const auto key = std::make_tuple(kernel, block, dyn_smem, ctx->device);
+  int per_cu = -1;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(key);
+    if (it != known.end()) per_cu = it->second;
+  }
+  if (per_cu < 0) {
+    EVD_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn_smem));
+    std::lock_guard<std::mutex> lk(mu);
+    known[key] = per_cu;
+  }
   *out = per_cu * ctx->num_cus;
   if (*out <= 0) return set_error(ctx, EVD_E_UNSUPPORTED, "kernel cannot be resident (block %d)", block);
   return 0;
@@ -264,6 +281,12 @@ int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value) {
       return 0;
     }
   return -2;
+}
+
+int evd_ctx_use_tune_cache(evd_ctx *ctx, int on) {
+  if (!ctx) return -1;
+  ctx->tune_cache = on != 0;
+  return 0;
 }
 
 int evd_ctx_set_precision(evd_ctx *ctx, int precision) {

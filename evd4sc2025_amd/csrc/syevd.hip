@@ -25,6 +25,7 @@ int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz);
 size_t stedc_workspace(long n);
 size_t bc_back_pack_bytes(long n, long b);
 size_t bc_back_wy_bytes(long n);
+bool tune_lookup(long n, long b, int precision, int *nb, Tuning *t);
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols);
@@ -249,15 +250,27 @@ extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, f
   EVD_ARG(ctx, vec || jobz == 'N' || jobz == 'n', 2);
   EVD_ARG(ctx, n >= 0, 3);
   EVD_ARG(ctx, b == 32 || b == 64 || (!vec && b >= 1 && b <= 64), 4);
-  EVD_ARG(ctx, nb >= b && nb % b == 0, 5);
+  EVD_ARG(ctx, nb == 0 || (nb >= b && nb % b == 0), 5);
   EVD_ARG(ctx, A != nullptr || n == 0, 6);
   EVD_ARG(ctx, lda >= std::max<long>(1, n), 7);
   EVD_ARG(ctx, w != nullptr || n == 0, 8);
   EVD_ARG(ctx, !vec || Z != nullptr || n == 0, 9);
   EVD_ARG(ctx, !vec || ldz >= std::max<long>(1, n), 10);
   if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_f32 cannot nest");
-  EVD_TRY(arena_ensure(ctx, syevd_workspace(n, b, nb, jobz, ctx->precision)));
-  return syevd(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+  // knobs left at 0 (and nb = 0) come from the tuning cache, else defaults
+  const Tuning user = ctx->tune;
+  int tnb = 0;
+  Tuning t;
+  if (ctx->tune_cache && tune_lookup(n, b, ctx->precision, &tnb, &t)) {
+    if (!ctx->tune.chase_groups) ctx->tune.chase_groups = t.chase_groups;
+    if (!ctx->tune.bc_valu) ctx->tune.bc_valu = t.bc_valu;
+    if (!ctx->tune.sbr_back_block) ctx->tune.sbr_back_block = t.sbr_back_block;
+  }
+  if (nb == 0) nb = tnb > 0 ? tnb : b * std::max<long>(1, 1024 / b);
+  int s = arena_ensure(ctx, syevd_workspace(n, b, nb, jobz, ctx->precision));
+  if (s == 0) s = syevd(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+  ctx->tune = user;
+  return s;
 }
 
 extern "C" int evd_orthogonality_f32(evd_ctx *ctx, long m, long n, const float *Q, long ldq, double *out) {

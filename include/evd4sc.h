@@ -175,11 +175,33 @@ int evd_sbr_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ld
  * destroyed).  b = bandwidth (32 or 64), nb = SBR block.  Replaces main's
  * SBR -> BC -> D&C -> back-transformation sequence (src/EVD/myEVD.cu:251-571). */
 int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz);
+/* (nb = 0: from the tuning cache, else 1024 rounded down to a multiple of b) */
 /* Arena bytes evd_syevd_f32 needs for (n, b, nb, jobz) in plain FP32. */
 size_t evd_syevd_workspace_f32(long n, long b, long nb, char jobz);
 /* The same for a given EVD_PREC_* (the FP16 form's back transformation keeps
  * its WY blocks, 16 KiB per 32 x 32 reflector block, in the arena). */
 size_t evd_syevd_workspace_ex(long n, long b, long nb, char jobz, int precision);
+
+/* ---- autotuning (no counterpart in the reference: its tile shapes are
+ * compile-time template arguments and the myEVD nb argument) ----
+ * Knob vector: {nb, chase_groups, bc_valu, sbr_back_block} (0 = default).
+ * evd_autotune solves a synthetic matrix of order n (device buffers of 3 n^2
+ * floats allocated for the call) for candidate knob values, one knob at a
+ * time, and stores the fastest setting in the process-wide cache under
+ * (next power of two >= n, b, the context's precision); knobs_out (nullable)
+ * receives it.  evd_syevd_f32 fills every knob the context leaves at 0, and
+ * nb = 0, from the cache entry of its (n, b, precision) when there is one.
+ * The cache is saved / loaded as text ("n_bucket b precision nb chase_groups
+ * bc_valu sbr_back_block" per line); load merges and rejects malformed files
+ * as a whole.  evd_tune_cache_get returns 1 when no entry exists. */
+int evd_autotune(evd_ctx *ctx, long n, long b, int reps, int *knobs_out);
+int evd_tune_cache_put(long n, long b, int precision, const int *knobs);
+int evd_tune_cache_get(long n, long b, int precision, int *knobs);
+int evd_tune_cache_clear(void);
+int evd_tune_cache_save(const char *path);
+int evd_tune_cache_load(const char *path);
+/* Whether evd_syevd_f32 on this context consults the cache (default 1). */
+int evd_ctx_use_tune_cache(evd_ctx *ctx, int on);
 
 /* ---- metrics (util/tool/checkMetric.cu:5-25, src/EVD/myEVD.cu:598-639) ---- */
 

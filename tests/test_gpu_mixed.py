@@ -19,6 +19,7 @@ def mctx():
     c = E.Context(0)
     yield c
     c.set_precision(E.PREC_FP32)
+    c.close()
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
