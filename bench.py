@@ -34,7 +34,7 @@ def profiled_traffic(kernel: str):
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated)."""
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("| `evd::" + kernel):
+            if line.startswith("| `") and kernel in line.split("`")[1]:
                 return float(line.strip().strip("|").split("|")[-1]) * 1e9
     except OSError:
         pass
@@ -170,18 +170,19 @@ def main():
     flops = (10.0 / 3.0) * n ** 3
     value = flops / dt / 1e12
 
-    # dominant kernel: the BC back transformation (bcback_kernel, timed alone
-    # with hipEvents on the solver's stream), 2 n^3 algorithmic flops (4 n per
-    # reflector entry), FP32 FMA on the VALU
+    # dominant kernel: the BC back transformation, WY blocks of 32 sweeps on
+    # FP32 MFMA (bcw_kernel, timed alone with hipEvents on the solver's
+    # stream); 2 n^3 algorithmic flops (4 n per reflector entry) - the MFMA
+    # issue 1.625x that, the staircase blocks' nonzero 16 x 16 tiles in full
     bc_ms = stage_ms["bc_back_kernel"]
     bc_flops = 2.0 * n ** 3
     achieved = bc_flops / (bc_ms * 1e-3) / 1e12
-    traffic = profiled_traffic("bcback_kernel") if (n, b) == (16384, 32) else None
+    traffic = profiled_traffic("bcw_kernel") if (n, b) == (16384, 32) else None
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": traffic, "kernel": "bcback_kernel",
-                "note": "2 n^3 flops per launch over the hipEvent time of the launch; FP32 peak 157.3 TF (the kernel "
-                        "runs FP32 FMA on the VALU); traffic = copy-calibrated FETCH_SIZE + WRITE_SIZE bytes per launch "
-                        "from the committed rocprofv3 summary"}
+                "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": traffic, "kernel": "bcw_kernel",
+                "note": "2 n^3 algorithmic flops per launch over the hipEvent time of the launch (the WY form issues "
+                        "1.625x that on v_mfma_f32_16x16x4f32); FP32 MFMA peak 157.3 TF; traffic = copy-calibrated "
+                        "FETCH_SIZE + WRITE_SIZE bytes per launch from the committed rocprofv3 summary"}
     # the bulge chase against HBM: (1.5 b + 1) n^2 FP64 window bytes moved per launch
     bc_bytes = (1.5 * b + 1) * n * n * 8
     chase = {"bound": "hbm", "achieved": bc_bytes / (stage_ms["bc"] * 1e-3) / 1e9 if stage_ms["bc"] else None,
