@@ -251,13 +251,15 @@ static int launch_chase(evd_ctx *ctx, long n, long b, double *band, long ldb, fl
   if (bytes >= (1L << 31))
     return set_error(ctx, EVD_E_UNSUPPORTED, "band of %ld bytes exceeds the 2 GiB buffer range of the chase", bytes);
   // workgroups: sweep s+1 trails sweep s by two b-row steps, so at most
-  // about n / (2b) sweeps are in flight; by default 1.25x that (a workgroup
-  // takes its next sweep by ticket as soon as it finishes one), at most the
-  // resident count.  Workgroups beyond that would only spin on progress words
-  // and crowd out concurrent solves.
+  // about n / (2b) sweeps are in flight; by default that many plus a few (a
+  // workgroup takes its next sweep by ticket as soon as it finishes one), at
+  // most the resident count.  Workgroups beyond that only spin on progress
+  // words, slow the chase through the polling traffic (n = 16384: 148 ms with
+  // 260 workgroups, 163 ms with 1.25x, 191 ms with all resident ones) and
+  // crowd out concurrent solves.
   int resident = 0;
   EVD_TRY(max_coresident_blocks(ctx, (const void *)chase_kernel<BW>, 256, 0, &resident));
-  const long inflight = (5 * ((n + 2 * b - 1) / (2 * b)) + 3) / 4 + 4;
+  const long inflight = (n + 2 * b - 1) / (2 * b) + 4;
   long G = ctx->tune.chase_groups > 0 ? ctx->tune.chase_groups
                                       : std::min<long>(inflight, (long)std::max(ctx->num_cus, resident));
   G = std::min<long>(G, std::max<long>(1, n - 2));

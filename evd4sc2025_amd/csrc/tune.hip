@@ -122,7 +122,8 @@ extern "C" int evd_autotune(evd_ctx *ctx, long n, long b, int reps, int *out_kno
       if (v >= b && v <= std::max<long>(256, n)) nbs.push_back(v);
     Knob knobs[] = {
         {&nb, nbs, -1, -1},
-        {&ctx->tune.chase_groups, {0, 128, 256, 512}, EVD_STAGE_BC, EVD_STAGE_BC + 1},
+        {&ctx->tune.chase_groups, {0, (int)(n / 128) + 4, (int)(3 * n / 256) + 4, (int)(n / 32) + 4}, EVD_STAGE_BC,
+         EVD_STAGE_BC + 1},
         {&ctx->tune.bc_valu, {0, 1}, EVD_STAGE_BCBACK, EVD_STAGE_BCBACK + 1},
         {&ctx->tune.sbr_back_block, {0, 512, 2048}, EVD_STAGE_SBRBACK, EVD_STAGE_SBRBACK + 1},
     };
