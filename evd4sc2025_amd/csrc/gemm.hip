@@ -27,10 +27,10 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBK = 16;
 
-template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T>
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, int BK = kBK>
 struct TileLoader {
-  static constexpr int NA = BM * kBK / 256;
-  static constexpr int NB = BN * kBK / 256;
+  static constexpr int NA = BM * BK / 256;
+  static constexpr int NB = BN * BK / 256;
   static constexpr int LDSA = BM + 4;
   static constexpr int LDSB = BN + 4;
   T ra[NA], rb[NB];
@@ -45,8 +45,8 @@ struct TileLoader {
         i = idx % BM;
         kk = idx / BM;
       } else {
-        kk = idx % kBK;
-        i = idx / kBK;
+        kk = idx % BK;
+        i = idx / BK;
       }
       int gi = m0 + i, gk = k0 + kk;
       T v = T(0);
@@ -61,8 +61,8 @@ struct TileLoader {
         j = idx % BN;
         kk = idx / BN;
       } else {
-        kk = idx % kBK;
-        j = idx / kBK;
+        kk = idx % BK;
+        j = idx / BK;
       }
       int gj = n0 + j, gk = k0 + kk;
       T v = T(0);
@@ -80,8 +80,8 @@ struct TileLoader {
         i = idx % BM;
         kk = idx / BM;
       } else {
-        kk = idx % kBK;
-        i = idx / kBK;
+        kk = idx % BK;
+        i = idx / BK;
       }
       As[kk * LDSA + i] = ra[q];
     }
@@ -93,8 +93,8 @@ struct TileLoader {
         j = idx % BN;
         kk = idx / BN;
       } else {
-        kk = idx % kBK;
-        j = idx / kBK;
+        kk = idx % BK;
+        j = idx / BK;
       }
       Bs[kk * LDSB + j] = rb[q];
     }
@@ -116,16 +116,16 @@ __device__ inline f64x4 mfma16(double a, double b, f64x4 c) {
 }
 
 // Core: accumulate op(A)[m0:m0+BM, kb:ke] * op(B)[kb:ke, n0:n0+BN] into acc.
-template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, typename VT>
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, typename VT, int BK = kBK>
 __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M,
                                      int N, int m0, int n0, int kb, int ke, T *As0, T *Bs0, VT (&acc)[BM / WM / 16][BN / WN / 16]) {
-  using L = TileLoader<TA, TB, BM, BN, WM, WN, T>;
+  using L = TileLoader<TA, TB, BM, BN, WM, WN, T, BK>;
   constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
-  constexpr int SA = kBK * L::LDSA, SB = kBK * L::LDSB;
+  constexpr int SA = BK * L::LDSA, SB = BK * L::LDSB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   L ld;
-  int nk = ke > kb ? (ke - kb + kBK - 1) / kBK : 0;
+  int nk = ke > kb ? (ke - kb + BK - 1) / BK : 0;
   if (nk > 0) {
     ld.load(A, lda, B, ldb, M, N, m0, n0, kb, ke, tid);
     ld.store(As0, Bs0, tid);
@@ -133,11 +133,11 @@ __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T 
   __syncthreads();
   for (int t = 0; t < nk; t++) {
     const int cur = t & 1;
-    if (t + 1 < nk) ld.load(A, lda, B, ldb, M, N, m0, n0, kb + (t + 1) * kBK, ke, tid);
+    if (t + 1 < nk) ld.load(A, lda, B, ldb, M, N, m0, n0, kb + (t + 1) * BK, ke, tid);
     const T *As = As0 + cur * SA;
     const T *Bs = Bs0 + cur * SB;
 #pragma unroll
-    for (int ks = 0; ks < kBK / 4; ks++) {
+    for (int ks = 0; ks < BK / 4; ks++) {
       T a[MI], b[NI];
       const int kr = ks * 4 + (lane >> 4);
 #pragma unroll
@@ -154,15 +154,15 @@ __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T 
   }
 }
 
-template <bool TA, bool TB, int BM, int BN, int WM, int WN>
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK>
 __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float alpha, const float *__restrict__ A,
                                                     long lda, long sA, const float *__restrict__ B, long ldb, long sB,
                                                     float beta, float *__restrict__ C, long ldc, long sC, int kchunk,
                                                     int ksplit, float *__restrict__ P, long batches) {
-  using L = TileLoader<TA, TB, BM, BN, WM, WN, float>;
+  using L = TileLoader<TA, TB, BM, BN, WM, WN, float, BK>;
   constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
-  __shared__ float As[2 * kBK * L::LDSA];
-  __shared__ float Bs[2 * kBK * L::LDSB];
+  __shared__ float As[2 * BK * L::LDSA];
+  __shared__ float Bs[2 * BK * L::LDSB];
   const long bz = blockIdx.z;
   const long batch = bz / ksplit;
   const int kz = (int)(bz % ksplit);
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
   for (int mi = 0; mi < MI; mi++)
 #pragma unroll
     for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
-  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4>(A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc);
+  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK>(A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -218,7 +218,7 @@ __global__ void splitk_reduce_kernel(int M, int N, long batches, int ksplit, con
   }
 }
 
-template <bool TA, bool TB, int BM, int BN, int WM, int WN>
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK>
 static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
                         const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
                         long min_chunk = 256) {
@@ -234,7 +234,7 @@ static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const
     if (ksplit < 1) ksplit = 1;
   }
   long kchunk = (k + ksplit - 1) / ksplit;
-  kchunk = (kchunk + kBK - 1) / kBK * kBK;
+  kchunk = (kchunk + BK - 1) / BK * BK;
   ksplit = (int)((k + kchunk - 1) / kchunk);
   if (ksplit < 1) ksplit = 1;
   float *P = nullptr;
@@ -244,7 +244,7 @@ static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const
     if (!P) return EVD_E_NOMEM;
   }
   dim3 grid((unsigned)((m + BM - 1) / BM), (unsigned)((n + BN - 1) / BN), (unsigned)(batch * ksplit));
-  sgemm_kernel<TA, TB, BM, BN, WM, WN><<<grid, 256, 0, ctx->stream>>>(
+  sgemm_kernel<TA, TB, BM, BN, WM, WN, BK><<<grid, 256, 0, ctx->stream>>>(
       (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch);
   EVD_LAUNCHED(ctx);
   if (ksplit > 1) {
@@ -321,10 +321,10 @@ int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, c
 __global__ __launch_bounds__(256) void syr2k_kernel(int n, int k, float alpha, const float *__restrict__ A, long lda,
                                                     const float *__restrict__ B, long ldb, float beta,
                                                     float *__restrict__ C, long ldc) {
-  constexpr int BM = 128, BN = 128, WM = 2, WN = 2, MI = 4, NI = 4;
-  using L = TileLoader<false, true, BM, BN, WM, WN, float>;
-  __shared__ float As[2 * kBK * L::LDSA];
-  __shared__ float Bs[2 * kBK * L::LDSB];
+  constexpr int BM = 128, BN = 128, WM = 2, WN = 2, MI = 4, NI = 4, BK = kBK;
+  using L = TileLoader<false, true, BM, BN, WM, WN, float, BK>;
+  __shared__ float As[2 * BK * L::LDSA];
+  __shared__ float Bs[2 * BK * L::LDSB];
   const long t = blockIdx.x;
   long ti = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
@@ -337,9 +337,9 @@ __global__ __launch_bounds__(256) void syr2k_kernel(int n, int k, float alpha, c
 #pragma unroll
     for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
   // A_I * B_J^T  (op(B)(kk, j) = B[j + kk*ldb])
-  tile_mainloop<false, true, BM, BN, WM, WN, float, f32x4>(A, lda, B, ldb, n, n, m0, n0, 0, k, As, Bs, acc);
+  tile_mainloop<false, true, BM, BN, WM, WN, float, f32x4, BK>(A, lda, B, ldb, n, n, m0, n0, 0, k, As, Bs, acc);
   // B_I * A_J^T
-  tile_mainloop<false, true, BM, BN, WM, WN, float, f32x4>(B, ldb, A, lda, n, n, m0, n0, 0, k, As, Bs, acc);
+  tile_mainloop<false, true, BM, BN, WM, WN, float, f32x4, BK>(B, ldb, A, lda, n, n, m0, n0, 0, k, As, Bs, acc);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
 #pragma unroll
