@@ -66,7 +66,8 @@ __host__ __device__ inline long wy_prefix(long Jn, long S) { return S * Jn - S *
 __host__ __device__ inline long wy_sweep_blocks(long nsweeps) { return (nsweeps + kB) / kB; }
 
 // ------------------------------------------------------------------ prep
-// One workgroup per block (S, j): V from U, T by the forward recurrence, V T,
+// One workgroup per block (S, j): V from U, T (forward recurrence, solved
+// column-parallel as T = (I + strict upper part of V^T V)^-1), V T,
 // and the MFMA operand fragments of V^T and -(V T), hi and lo halves.
 // A block is 16 fragments of 64 lanes x 16 bytes.
 // Split FP16 (h8 per lane, K = 32 per MFMA):
@@ -95,7 +96,6 @@ __global__ __launch_bounds__(256) void bcw_prep_kernel(int n, const float *__res
     if (i >= 0 && i < kB && sw >= 0 && sw < nsweeps && row < n) v = (double)U[row + sw * ldu];
     V[r][t] = v;
   }
-  for (int idx = tid; idx < 32 * 32; idx += 256) T[idx & 31][idx >> 5] = ((idx & 31) == (idx >> 5)) ? 1.0 : 0.0;
   __syncthreads();
   // G[a][c] = v_a . v_c for a < c: the columns overlap in rows c .. a+31
   for (int idx = tid; idx < 32 * 32; idx += 256) {
@@ -106,14 +106,18 @@ __global__ __launch_bounds__(256) void bcw_prep_kernel(int n, const float *__res
     G[a][c] = s;
   }
   __syncthreads();
-  for (int t = 1; t < kB; t++) {
-    if (tid < t) {
+  // With tau = 1 the recurrence gives T = (I + strict upper part of G)^-1:
+  // column c by back substitution, one thread per column, no barriers.
+  if (tid < kB) {
+    const int c = tid;
+    for (int i = 0; i < kB; i++) T[i][c] = (i == c) ? 1.0 : 0.0;
+    for (int i = c - 1; i >= 0; i--) {
       double s = 0.0;
-      for (int q = tid; q < t; q++) s += T[tid][q] * G[q][t];
-      T[tid][t] = -s;
+      for (int j = i + 1; j <= c; j++) s += G[i][j] * T[j][c];
+      T[i][c] = -s;
     }
-    __syncthreads();
   }
+  __syncthreads();
   for (int idx = tid; idx < 64 * 32; idx += 256) {
     const int r = idx & 63, t = idx >> 6;
     double s = 0.0;

@@ -31,6 +31,10 @@
 
 namespace evd {
 
+// one progress word per 128-byte line: the word a sweep publishes is polled by
+// its successor only, and words sharing a line would serialise unrelated
+// sweeps' polls and stores on that line
+constexpr long kComStride = 32;
 constexpr unsigned kBufferWord3 = 0x00020000u;  // raw buffer descriptor: 32-bit data format
 
 constexpr int kSc1 = 16;  // buffer-op cache policy: sc1, device-coherent
@@ -91,7 +95,7 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
       s_ok = 1;
       if (s > 0) {
         const unsigned long long t0 = clock100();
-        while (ld_dev(&com[s - 1]) < need) {
+        while (ld_dev(&com[(long)(s - 1) * kComStride]) < need) {
           __builtin_amdgcn_s_sleep(1);
           if (clock100() - t0 > kWatchdogTicks) {
             atomicOr(err, (unsigned)EVD_E_TIMEOUT);
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
       }
       wait_mem();
       __syncthreads();
-      if (tid == 0) st_dev(&com[s], last ? INT_MAX : p + len);
+      if (tid == 0) st_dev(&com[(long)s * kComStride], last ? INT_MAX : p + len);
       if (last) break;
       double *t = B1;
       B1 = Bn;
@@ -210,7 +214,7 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
       ok = wait_prev(s, p + 2 * b - 1);
     }
     if (!ok) {
-      if (tid == 0) st_dev(&com[s], INT_MAX);
+      if (tid == 0) st_dev(&com[(long)s * kComStride], INT_MAX);
       return;
     }
   }
@@ -276,11 +280,11 @@ int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, lon
   EVD_TRY(fill_f32(ctx, U, ldu, n, ldu, 0.f));
   if (n > 2) {
     ArenaFrame frame(ctx);
-    // progress words com[0..n), then the sweep ticket
-    int *com = arena_alloc_n<int>(ctx, (size_t)n + 1);
+    // progress words com[s * kComStride] for sweeps 0..n-1, then the sweep ticket
+    int *com = arena_alloc_n<int>(ctx, ((size_t)n + 1) * kComStride);
     if (!com) return EVD_E_NOMEM;
-    unsigned *ticket = reinterpret_cast<unsigned *>(com + n);
-    EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), (size_t)n + 1, 0u));
+    unsigned *ticket = reinterpret_cast<unsigned *>(com + (size_t)n * kComStride);
+    EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), ((size_t)n + 1) * kComStride, 0u));
     if (b <= 32)
       EVD_TRY(launch_chase<32>(ctx, n, b, band, ldband, U, ldu, com, ticket));
     else if (b <= 64)
@@ -314,7 +318,7 @@ extern "C" int evd_sb2st_f64(evd_ctx *ctx, long n, long b, double *band, long ld
   EVD_ARG(ctx, b >= 1 && b <= 64, 3);
   EVD_ARG(ctx, ldband >= 2 * b, 5);
   EVD_ARG(ctx, ldu >= n + b, 7);
-  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)(n + 2) * sizeof(int) + 4096));
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)(n + 2) * kComStride * sizeof(int) + 4096));
   EVD_TRY(sb2st(ctx, n, b, band, ldband, U, ldu, D, E));
   return check_device_error(ctx, "evd_sb2st_f64");
 }
