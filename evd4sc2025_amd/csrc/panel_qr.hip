@@ -360,8 +360,13 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
   unsigned *err = ctx->ctrl + kCtrlError;
   int mi = (int)m, ni = (int)nc;
   void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &tau_out, &tags, &epoch, &err};
-  EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0,
-                                          ctx->stream));
+  // one workgroup is trivially co-resident: a regular launch, which (unlike a
+  // cooperative one) may overlap with the work of other streams
+  if (G == 1)
+    EVD_HIP(ctx, hipLaunchKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0, ctx->stream));
+  else
+    EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0,
+                                            ctx->stream));
   return 0;
 }
 
