@@ -22,7 +22,7 @@ __all__ = [
     "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "hou_tsqr_panel", "tc_ozimmu_syr2k", "band_pack",
     "sb2st", "stedc", "computerWYFromSlide", "computerQFromWY", "my_BC_back_trans_v8_10_noBandU", "bc_back",
     "sbr_back", "checkOrthogonality", "backward_error", "sgemm", "dgemm", "gen_symmetric", "gen_spectrum",
-    "STAGES", "autotune", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
+    "STAGES", "autotune", "syevd_batched", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
 ]
 
 # GEMM precision of SBR and the back transformation (evd_ctx_set_precision)
@@ -373,6 +373,28 @@ def syevd(ctx: Context, A, b: int = 32, nb: int = 1024, jobz: str = "V", overwri
 
 def syevd_workspace(n: int, b: int, nb: int, jobz: str = "V") -> int:
     return int(lib().evd_syevd_workspace_f32(n, b, nb, _ch(jobz)))
+
+
+def syevd_batched(ctx: Context, mats, b: int = 32, nb: int = 0, jobz: str = "V"):
+    """Same-size batch (evd_syevd_batched_f32): `mats` is a list of n x n CUDA
+    tensors (lower triangles read).  Returns (w, Z): w of shape (count, n)
+    (float64) and Z of shape (count, n, n) with Z[q] the eigenvectors of
+    mats[q] (columns), each identical to a single syevd of that matrix."""
+    torch = _torch()
+    count = len(mats)
+    n = mats[0].shape[0]
+    dev = mats[0].device
+    # column-major storage of matrix q = row-major storage of its transpose
+    A = torch.empty((count, n, n), dtype=torch.float32, device=dev)
+    for q, M in enumerate(mats):
+        A[q].copy_(M.t())
+    w = torch.empty((count, max(n, 1)), dtype=torch.float64, device=dev)
+    Zc = torch.empty((count, n, n), dtype=torch.float32, device=dev) if jobz == "V" else None
+    ctx.call("evd_syevd_batched_f32", _ch(jobz), count, n, b, nb, C.c_void_p(A.data_ptr()), max(n, 1), n * n,
+             C.c_void_p(w.data_ptr()), max(n, 1), C.c_void_p(Zc.data_ptr() if Zc is not None else 0), max(n, 1),
+             n * n)
+    Z = Zc.transpose(1, 2) if Zc is not None else None
+    return w[:, :n], Z
 
 
 # ---- autotuning (evd_autotune / evd_tune_cache_* in the C ABI)

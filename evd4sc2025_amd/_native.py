@@ -48,6 +48,9 @@ SIGNATURES = {
     "evd_syevd_workspace_f32": (_sz, [_l, _l, _l, _ch]),
     "evd_syevd_workspace_ex": (_sz, [_l, _l, _l, _ch, _i]),
     "evd_autotune": (_i, [_vp, _l, _l, _i, C.POINTER(_i)]),
+    "evd_syevd_batched_f32": (_i, [_vp, _ch, _i, _l, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l, _l]),
+    "evd_syevd_batched_workspace_f32": (_sz, [_i, _l, _l, _l, _ch, _i]),
+    "evd_sb2st_batched_f64": (_i, [_vp, _i, _l, _l, _vp, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l]),
     "evd_tune_cache_put": (_i, [_l, _l, _i, C.POINTER(_i)]),
     "evd_tune_cache_get": (_i, [_l, _l, _i, C.POINTER(_i)]),
     "evd_tune_cache_clear": (_i, []),

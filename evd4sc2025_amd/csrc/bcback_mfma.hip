@@ -79,7 +79,10 @@ __host__ __device__ inline long wy_sweep_blocks(long nsweeps) { return (nsweeps 
 //   [8..15] -V T: fragment 8 + 2 mt + st (mt: 16-row window tile, st: 16-sweep tile)
 template <bool F32>
 __global__ __launch_bounds__(256) void bcw_prep_kernel(int n, const float *__restrict__ U, long ldu, int nsweeps,
-                                                       h8 *__restrict__ F, unsigned *__restrict__ err) {
+                                                       h8 *__restrict__ F, unsigned *__restrict__ err, long u_stride,
+                                                       long f_stride) {
+  U += blockIdx.z * u_stride;  // problem blockIdx.z of a batch
+  F += blockIdx.z * f_stride;
   const long Jn = (n + kB - 1) / kB;
   const int S = blockIdx.y, j = blockIdx.x;
   if (j >= wy_steps(Jn, S)) return;
@@ -395,9 +398,11 @@ struct HalfIO {
 
 template <int NW, bool F32>
 __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *__restrict__ Z, long ldz,
-                                                      const h8 *__restrict__ F16, int nsweeps) {
+                                                      const h8 *__restrict__ F16, int nsweeps, long z_stride,
+                                                      long f_stride) {
   typedef typename std::conditional<F32, f4, h8>::type Frag;
-  const Frag *F = reinterpret_cast<const Frag *>(F16);
+  Z += blockIdx.y * z_stride;  // problem blockIdx.y of a batch
+  const Frag *F = reinterpret_cast<const Frag *>(F16 + blockIdx.y * f_stride);
   static_assert(NW == 8, "each wave moves 8 of the 64 columns of a half window");
   // hand-off slots (2 per wave), the rows wave 0 takes in at each step, and
   // the top rows of each wave's first step (both double-buffered): 160 KiB
@@ -491,30 +496,43 @@ size_t bc_back_wy_bytes(long n) {
 bool bc_back_wy_fits(long n, long ldq) { return (63 * ldq + n) * 4 < (1L << 31); }
 
 // Q <- H_1 .. H_K Q for b = 32 as WY blocks on the matrix cores: FP32 MFMA, or
-// split FP16 (split16: columns of Q with 2-norm <= 8)
-int bc_back_wy(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu, bool split16) {
+// split FP16 (split16: columns of Q with 2-norm <= 8).  A batch of `count`
+// same-size problems (Q + q q_stride, U + q u_stride) runs as one launch of
+// each kernel.
+int bc_back_wy_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride, const float *U,
+                       long ldu, long u_stride, bool split16) {
   const int nsweeps = (int)std::max<long>(0, n - 2);
-  if (nsweeps == 0 || ncols <= 0) return 0;
+  if (nsweeps == 0 || ncols <= 0 || count <= 0) return 0;
   ArenaFrame frame(ctx);
-  h8 *F = static_cast<h8 *>(arena_alloc(ctx, bc_back_wy_bytes(n)));
+  const long fbytes = (long)bc_back_wy_bytes(n);
+  h8 *F = static_cast<h8 *>(arena_alloc(ctx, (size_t)fbytes * count));
   if (!F) return EVD_E_NOMEM;
+  const long f_stride = fbytes / (long)sizeof(h8);
   const long Jn = (n + kB - 1) / kB, nblk = wy_sweep_blocks(nsweeps);
-  const dim3 pg((unsigned)Jn, (unsigned)nblk);
+  const dim3 pg((unsigned)Jn, (unsigned)nblk, (unsigned)count);
   if (split16)
-    bcw_prep_kernel<false><<<pg, 256, 0, ctx->stream>>>((int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError);
+    bcw_prep_kernel<false><<<pg, 256, 0, ctx->stream>>>((int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError, u_stride,
+                                                         f_stride);
   else
-    bcw_prep_kernel<true><<<pg, 256, 0, ctx->stream>>>((int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError);
+    bcw_prep_kernel<true><<<pg, 256, 0, ctx->stream>>>((int)n, U, ldu, nsweeps, F, ctx->ctrl + kCtrlError, u_stride,
+                                                        f_stride);
   EVD_LAUNCHED(ctx);
   if (ctx->timing) hipEventRecord(ctx->kev[0], ctx->stream);
   constexpr int NW = 8;
-  const dim3 grid((unsigned)((ncols + 63) / 64));
+  const dim3 grid((unsigned)((ncols + 63) / 64), (unsigned)count);
   if (split16)
-    bcw_kernel<NW, false><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F, nsweeps);
+    bcw_kernel<NW, false><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F, nsweeps, q_stride,
+                                                              f_stride);
   else
-    bcw_kernel<NW, true><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F, nsweeps);
+    bcw_kernel<NW, true><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F, nsweeps, q_stride,
+                                                             f_stride);
   EVD_LAUNCHED(ctx);
   if (ctx->timing) hipEventRecord(ctx->kev[1], ctx->stream);
   return 0;
+}
+
+int bc_back_wy(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu, bool split16) {
+  return bc_back_wy_batched(ctx, 1, n, ncols, Q, ldq, 0, U, ldu, 0, split16);
 }
 
 }  // namespace evd

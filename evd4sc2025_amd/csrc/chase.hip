@@ -69,11 +69,16 @@ __device__ inline void chase_reflector(const double *Bc, int lenx, int lane, dou
   if (lane == 0) *s_beta = (nrm > 0.0) ? (x0 >= 0.0 ? -nrm : nrm) : x0;
 }
 
+// A batch of `count` same-size problems shares one launch: ticket t is sweep
+// t / count of problem t % count, so each problem's sweeps are still handed
+// out in increasing order.  Problem q's band, reflectors and progress words
+// start at band + q * band_stride, U + q * u_stride, com + q * com_stride.
 template <int BW>
-__global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int b, double *__restrict__ band, long ldb,
-                                                    unsigned band_bytes, float *__restrict__ U, long ldu,
-                                                    int *__restrict__ com, unsigned *__restrict__ ticket,
-                                                    unsigned *__restrict__ err) {
+__global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int b, double *__restrict__ band_all,
+                                                    long ldb, unsigned band_bytes, float *__restrict__ U_all,
+                                                    long ldu, int *__restrict__ com_all,
+                                                    unsigned *__restrict__ ticket, unsigned *__restrict__ err,
+                                                    int count, long band_stride, long u_stride, long com_stride) {
   constexpr int LD = BW + 1;
   constexpr int LPR = 256 / BW;  // lanes per window row
   constexpr int EPT = BW / LPR;  // window columns per lane
@@ -81,16 +86,14 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
   __shared__ double sB[2][BW * LD];
   __shared__ double u[BW], yv[BW], qv[BW];
   __shared__ double s_beta;
-  __shared__ int s_sweep, s_ok;
+  __shared__ int s_ticket, s_ok;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ri = tid / LPR, kc = tid % LPR;
   const int nsweeps = n - 2;
   const unsigned ld = (unsigned)ldb;  // band offsets fit 32 bits (checked by the launcher)
 
-  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(band, 0, (int)band_bytes, kBufferWord3);
-
   // tid 0 polls the predecessor's progress word; false = watchdog expired
-  auto wait_prev = [&](int s, int need) -> bool {
+  auto wait_prev = [&](const int *com, int s, int need) -> bool {
     if (tid == 0) {
       s_ok = 1;
       if (s > 0) {
@@ -111,17 +114,22 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
 
   while (true) {
     if (tid == 0)
-      s_sweep = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_ticket = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const int s = s_sweep;
+    const int tk = s_ticket;
     __syncthreads();
-    if (s >= nsweeps) return;
+    if (tk >= nsweeps * count) return;
+    const int q = tk % count, s = tk / count;
+    const __amdgpu_buffer_rsrc_t rb =
+        __builtin_amdgcn_make_buffer_rsrc(band_all + (long)q * band_stride, 0, (int)band_bytes, kBufferWord3);
+    float *U = U_all + (long)q * u_stride;
+    int *com = com_all + (long)q * com_stride;
     int p = s + 1;
     int len = min(b, n - p);
     int colB1 = 1, cB1 = s;
     bool first = true;
     double *B1 = sB[0], *Bn = sB[1];
-    bool ok = wait_prev(s, p + 2 * b - 1);
+    bool ok = wait_prev(com, s, p + 2 * b - 1);
     if (ok) {
       // column s below the diagonal is final: it becomes B1[:, 0]
       for (int i = tid; i < BW; i += 256) B1[i * LD] = (i < len) ? ld_band(rb, (unsigned)(1 + i) + (unsigned)s * ld) : 0.0;
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
       p += len;
       len = len2;
       first = false;
-      ok = wait_prev(s, p + 2 * b - 1);
+      ok = wait_prev(com, s, p + 2 * b - 1);
     }
     if (!ok) {
       if (tid == 0) st_dev(&com[(long)s * kComStride], INT_MAX);
@@ -231,14 +239,6 @@ __global__ void band_pack_kernel(int n, int b, const float *__restrict__ A, long
     }
 }
 
-__global__ void band_extract_kernel(int n, const double *__restrict__ band, long ldb, double *__restrict__ D,
-                                    double *__restrict__ E) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    D[i] = band[i * ldb];
-    if (i < n - 1) E[i] = band[1 + i * ldb];
-  }
-}
-
 int band_pack(evd_ctx *ctx, long n, long b, const float *A, long lda, double *band, long ldband) {
   if (n <= 0) return 0;
   dim3 blk(64, 4);
@@ -249,53 +249,78 @@ int band_pack(evd_ctx *ctx, long n, long b, const float *A, long lda, double *ba
 }
 
 template <int BW>
-static int launch_chase(evd_ctx *ctx, long n, long b, double *band, long ldb, float *U, long ldu, int *com,
-                        unsigned *ticket) {
+static int launch_chase(evd_ctx *ctx, int count, long n, long b, double *band, long ldb, long band_stride, float *U,
+                        long ldu, long u_stride, int *com, long com_stride, unsigned *ticket) {
   const long bytes = ldb * n * (long)sizeof(double);
   if (bytes >= (1L << 31))
     return set_error(ctx, EVD_E_UNSUPPORTED, "band of %ld bytes exceeds the 2 GiB buffer range of the chase", bytes);
   // workgroups: sweep s+1 trails sweep s by two b-row steps, so at most
-  // about n / (2b) sweeps are in flight; by default that many plus a few (a
-  // workgroup takes its next sweep by ticket as soon as it finishes one), at
-  // most the resident count.  Workgroups beyond that only spin on progress
-  // words, slow the chase through the polling traffic (n = 16384: 148 ms with
-  // 260 workgroups, 163 ms with 1.25x, 191 ms with all resident ones) and
-  // crowd out concurrent solves.
+  // about n / (2b) sweeps of a problem are in flight; by default that many
+  // plus a few per problem (a workgroup takes its next sweep by ticket as
+  // soon as it finishes one), at most the resident count.  Workgroups beyond
+  // that only spin on progress words, slow the chase through the polling
+  // traffic (n = 16384: 148 ms with 260 workgroups, 163 ms with 1.25x,
+  // 191 ms with all resident ones) and crowd out concurrent solves.
   int resident = 0;
   EVD_TRY(max_coresident_blocks(ctx, (const void *)chase_kernel<BW>, 256, 0, &resident));
-  const long inflight = (n + 2 * b - 1) / (2 * b) + 4;
+  const long inflight = ((n + 2 * b - 1) / (2 * b) + 4) * count;
   long G = ctx->tune.chase_groups > 0 ? ctx->tune.chase_groups
                                       : std::min<long>(inflight, (long)std::max(ctx->num_cus, resident));
-  G = std::min<long>(G, std::max<long>(1, n - 2));
+  G = std::min<long>(G, std::max<long>(1, (n - 2) * count));
   int ni = (int)n, bi = (int)b;
   unsigned nbytes = (unsigned)bytes;
   unsigned *err = ctx->ctrl + kCtrlError;
-  chase_kernel<BW><<<(unsigned)G, 256, 0, ctx->stream>>>(ni, bi, band, ldb, nbytes, U, ldu, com, ticket, err);
+  chase_kernel<BW><<<(unsigned)G, 256, 0, ctx->stream>>>(ni, bi, band, ldb, nbytes, U, ldu, com, ticket, err, count,
+                                                          band_stride, u_stride, com_stride);
   EVD_LAUNCHED(ctx);
   return 0;
 }
 
-int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E) {
-  if (n <= 0) return 0;
-  EVD_TRY(fill_f32(ctx, U, ldu, n, ldu, 0.f));
+__global__ void band_extract_batched_kernel(int n, const double *__restrict__ band, long ldb, long band_stride,
+                                            double *__restrict__ D, long d_stride, double *__restrict__ E,
+                                            long e_stride) {
+  const long q = blockIdx.y;
+  band += q * band_stride;
+  D += q * d_stride;
+  E += q * e_stride;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    D[i] = band[i * ldb];
+    if (i < n - 1) E[i] = band[1 + i * ldb];
+  }
+}
+
+// Bulge chasing of `count` same-size bands in one persistent launch (problem q
+// at band + q band_stride, U + q u_stride, D + q d_stride, E + q e_stride).
+int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride, float *U,
+                  long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride) {
+  if (n <= 0 || count <= 0) return 0;
+  for (int q = 0; q < count; q++) EVD_TRY(fill_f32(ctx, U + (long)q * u_stride, ldu, n, ldu, 0.f));
   if (n > 2) {
     ArenaFrame frame(ctx);
-    // progress words com[s * kComStride] for sweeps 0..n-1, then the sweep ticket
-    int *com = arena_alloc_n<int>(ctx, ((size_t)n + 1) * kComStride);
+    // progress words com[q * n * kComStride + s * kComStride], then the ticket
+    const long com_stride = n * kComStride;
+    const size_t words = (size_t)count * com_stride + 1;
+    int *com = arena_alloc_n<int>(ctx, words);
     if (!com) return EVD_E_NOMEM;
-    unsigned *ticket = reinterpret_cast<unsigned *>(com + (size_t)n * kComStride);
-    EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), ((size_t)n + 1) * kComStride, 0u));
+    unsigned *ticket = reinterpret_cast<unsigned *>(com + (size_t)count * com_stride);
+    EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), words, 0u));
     if (b <= 32)
-      EVD_TRY(launch_chase<32>(ctx, n, b, band, ldband, U, ldu, com, ticket));
+      EVD_TRY(launch_chase<32>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride, ticket));
     else if (b <= 64)
-      EVD_TRY(launch_chase<64>(ctx, n, b, band, ldband, U, ldu, com, ticket));
+      EVD_TRY(launch_chase<64>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride, ticket));
     else
       return set_error(ctx, EVD_E_UNSUPPORTED, "bandwidth %ld > 64", b);
   }
-  band_extract_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>((int)n, band, ldband,
-                                                                                               D, E);
+  band_extract_batched_kernel<<<dim3((unsigned)std::min<long>((n + 255) / 256, 1024), (unsigned)count), 256, 0,
+                                ctx->stream>>>((int)n, band, ldband, band_stride, D, d_stride, E, e_stride);
   EVD_LAUNCHED(ctx);
   return 0;
+}
+
+size_t sb2st_batched_workspace(int count, long n) { return align256(((size_t)count * n * kComStride + 1) * sizeof(int)); }
+
+int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E) {
+  return sb2st_batched(ctx, 1, n, b, band, ldband, 0, U, ldu, 0, D, 0, E, 0);
 }
 
 }  // namespace evd
@@ -318,7 +343,26 @@ extern "C" int evd_sb2st_f64(evd_ctx *ctx, long n, long b, double *band, long ld
   EVD_ARG(ctx, b >= 1 && b <= 64, 3);
   EVD_ARG(ctx, ldband >= 2 * b, 5);
   EVD_ARG(ctx, ldu >= n + b, 7);
-  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, (size_t)(n + 2) * kComStride * sizeof(int) + 4096));
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, sb2st_batched_workspace(1, n) + 4096));
   EVD_TRY(sb2st(ctx, n, b, band, ldband, U, ldu, D, E));
   return check_device_error(ctx, "evd_sb2st_f64");
+}
+
+extern "C" int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, double *band, long ldband,
+                                     long band_stride, float *U, long ldu, long u_stride, double *D, long d_stride,
+                                     double *E, long e_stride) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, count >= 0, 2);
+  EVD_ARG(ctx, n >= 0, 3);
+  EVD_ARG(ctx, b >= 1 && b <= 64, 4);
+  EVD_ARG(ctx, ldband >= 2 * b, 6);
+  EVD_ARG(ctx, count <= 1 || band_stride >= ldband * n, 7);
+  EVD_ARG(ctx, ldu >= n + b, 9);
+  EVD_ARG(ctx, count <= 1 || u_stride >= ldu * n, 10);
+  EVD_ARG(ctx, count <= 1 || d_stride >= n, 12);
+  EVD_ARG(ctx, count <= 1 || e_stride >= std::max<long>(1, n - 1), 14);
+  EVD_ARG(ctx, (long)count * std::max<long>(n, 1) < (1L << 30), 2);
+  if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, sb2st_batched_workspace(count, n) + 4096));
+  EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, d_stride, E, e_stride));
+  return check_device_error(ctx, "evd_sb2st_batched_f64");
 }

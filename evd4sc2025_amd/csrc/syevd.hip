@@ -26,6 +26,12 @@ size_t stedc_workspace(long n);
 size_t bc_back_pack_bytes(long n, long b);
 size_t bc_back_wy_bytes(long n);
 bool tune_lookup(long n, long b, int precision, int *nb, Tuning *t);
+int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride, float *U,
+                  long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride);
+size_t sb2st_batched_workspace(int count, long n);
+bool bc_back_wy_fits(long n, long ldq);
+int bc_back_wy_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride, const float *U,
+                       long ldu, long u_stride, bool split16);
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols);
@@ -194,6 +200,104 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
   return 0;
 }
 
+// ------------------------------------------------------------- batched
+// `count` same-size problems (A + q a_stride, w + q w_stride, Z + q z_stride).
+// The bulge chase of all problems is one persistent launch (tickets
+// interleave the problems' sweeps) and so is the BC back transformation (one
+// WY prep and one WY kernel launch, problem = grid y); SBR, the D&C and the
+// SBR back transformation run per problem in order on the context stream.
+size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision) {
+  const long wc = wy_cols(n);
+  const bool vec = (jobz == 'V' || jobz == 'v');
+  size_t per = align256((size_t)n * wc * sizeof(float)) * 2 + align256((size_t)2 * b * n * sizeof(double)) +
+               align256((size_t)(n + b) * n * sizeof(float)) + align256((size_t)n * sizeof(double)) * 2;
+  size_t s = per * count + 4 * 256;
+  size_t later = std::max(sy2sb_workspace(n, b, nb), sb2st_batched_workspace(count, n) + ((size_t)1 << 20));
+  if (vec) {
+    s += align256((size_t)n * n * sizeof(double));
+    size_t back = syevd_workspace(n, b, nb, jobz, precision);  // per-problem back transformation and D&C bound
+    if (b == 32) back = std::max(back, bc_back_wy_bytes(n) * count + ((size_t)1 << 20));
+    later = std::max(later, back);
+  }
+  return s + later + ((size_t)4 << 20);
+}
+
+int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda, long a_stride,
+                  double *w, long w_stride, float *Z, long ldz, long z_stride) {
+  const bool vec = (jobz == 'V' || jobz == 'v');
+  if (n == 0 || count == 0) return 0;
+  ArenaFrame frame(ctx);
+  const long wc = wy_cols(n);
+  std::vector<float *> W(count), Y(count);
+  for (int q = 0; q < count; q++) {
+    W[q] = arena_alloc_n<float>(ctx, (size_t)n * wc);
+    Y[q] = arena_alloc_n<float>(ctx, (size_t)n * wc);
+    if (!W[q] || !Y[q]) return EVD_E_NOMEM;
+  }
+  const long ldband = 2 * b, band_stride = ldband * n, ldu = n + b, u_stride = ldu * n;
+  const long e_stride = std::max<long>(n, 1);
+  double *band = arena_alloc_n<double>(ctx, (size_t)band_stride * count);
+  float *U = arena_alloc_n<float>(ctx, (size_t)u_stride * count);
+  double *D = arena_alloc_n<double>(ctx, (size_t)n * count);
+  double *E = arena_alloc_n<double>(ctx, (size_t)e_stride * count);
+  double *Zt = vec ? arena_alloc_n<double>(ctx, (size_t)n * n) : nullptr;
+  if (!band || !U || !D || !E || (vec && !Zt)) return EVD_E_NOMEM;
+
+  for (int q = 0; q < count; q++) {
+    symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A + q * a_stride, lda, ctx->ctrl + kCtrlError);
+    EVD_LAUNCHED(ctx);
+  }
+  EVD_TRY(check_device_error(ctx, "input screen"));
+
+  stage_mark(ctx, 0);
+  for (int q = 0; q < count; q++) {
+    float *Aq = A + q * a_stride;
+    EVD_TRY(sy2sb(ctx, n, b, nb, Aq, lda, W[q], n, Y[q], n, wc));
+    EVD_TRY(band_pack(ctx, n, b, Aq, lda, band + q * band_stride, ldband));
+  }
+  stage_mark(ctx, 1);
+  EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, n, E, e_stride));
+  stage_mark(ctx, 2);
+  if (vec) {
+    for (int q = 0; q < count; q++) {
+      EVD_TRY(stedc(ctx, n, D + q * n, E + q * e_stride, Zt, n));
+      copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D + q * n,
+                                                                                               w + q * w_stride);
+      EVD_LAUNCHED(ctx);
+      f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt, n, Z + q * z_stride, ldz);
+      EVD_LAUNCHED(ctx);
+    }
+    stage_mark(ctx, 3);
+    if (b == 32 && !ctx->tune.bc_valu && bc_back_wy_fits(n, ldz)) {
+      EVD_TRY(bc_back_wy_batched(ctx, count, n, n, Z, ldz, z_stride, U, ldu, u_stride,
+                                 ctx->precision == EVD_PREC_FP16));
+    } else {
+      for (int q = 0; q < count; q++)
+        EVD_TRY(bc_back(ctx, n, b, n, Z + q * z_stride, ldz, U + q * u_stride, ldu, true));
+    }
+    stage_mark(ctx, 4);
+    for (int q = 0; q < count; q++) EVD_TRY(sbr_back(ctx, n, b, n, Z + q * z_stride, ldz, W[q], n, Y[q], n, wc));
+    stage_mark(ctx, 5);
+  } else {
+    for (int q = 0; q < count; q++) EVD_TRY(sterf_bisect(ctx, n, D + q * n, E + q * e_stride, w + q * w_stride));
+    stage_mark(ctx, 3);
+    stage_mark(ctx, 4);
+    stage_mark(ctx, 5);
+  }
+  EVD_TRY(check_device_error(ctx, "evd_syevd_batched_f32"));
+  if (ctx->timing) {
+    for (int i = 0; i < EVD_STAGE_COUNT; i++) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]);
+      ctx->stage_ms[i] = ms;
+    }
+    float km = 0.f;
+    if (vec) hipEventElapsedTime(&km, ctx->kev[0], ctx->kev[1]);
+    ctx->stage_ms[EVD_TIMER_BCBACK_KERNEL] = km;
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------- metrics
 __global__ void frob_sq_kernel(long m, long n, const double *__restrict__ X, long ldx, int minus_identity,
                                double *__restrict__ out) {
@@ -269,6 +373,44 @@ extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, f
   if (nb == 0) nb = tnb > 0 ? tnb : b * std::max<long>(1, 1024 / b);
   int s = arena_ensure(ctx, syevd_workspace(n, b, nb, jobz, ctx->precision));
   if (s == 0) s = syevd(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+  ctx->tune = user;
+  return s;
+}
+
+extern "C" size_t evd_syevd_batched_workspace_f32(int count, long n, long b, long nb, char jobz, int precision) {
+  if (n <= 0 || count <= 0) return 0;
+  return syevd_batched_workspace(count, n, b, nb == 0 ? b * std::max<long>(1, 1024 / b) : nb, jobz, precision);
+}
+
+extern "C" int evd_syevd_batched_f32(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda,
+                                     long a_stride, double *w, long w_stride, float *Z, long ldz, long z_stride) {
+  if (!ctx) return -1;
+  const bool vec = (jobz == 'V' || jobz == 'v');
+  EVD_ARG(ctx, vec || jobz == 'N' || jobz == 'n', 2);
+  EVD_ARG(ctx, count >= 0, 3);
+  EVD_ARG(ctx, n >= 0 && (long)count * n < (1L << 30), 4);
+  EVD_ARG(ctx, b == 32 || b == 64 || (!vec && b >= 1 && b <= 64), 5);
+  EVD_ARG(ctx, nb == 0 || (nb >= b && nb % b == 0), 6);
+  EVD_ARG(ctx, A != nullptr || n == 0 || count == 0, 7);
+  EVD_ARG(ctx, lda >= std::max<long>(1, n), 8);
+  EVD_ARG(ctx, count <= 1 || a_stride >= lda * n, 9);
+  EVD_ARG(ctx, w != nullptr || n == 0 || count == 0, 10);
+  EVD_ARG(ctx, count <= 1 || w_stride >= n, 11);
+  EVD_ARG(ctx, !vec || Z != nullptr || n == 0 || count == 0, 12);
+  EVD_ARG(ctx, !vec || ldz >= std::max<long>(1, n), 13);
+  EVD_ARG(ctx, !vec || count <= 1 || z_stride >= ldz * n, 14);
+  if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_batched_f32 cannot nest");
+  const Tuning user = ctx->tune;
+  int tnb = 0;
+  Tuning t;
+  if (ctx->tune_cache && tune_lookup(n, b, ctx->precision, &tnb, &t)) {
+    if (!ctx->tune.chase_groups) ctx->tune.chase_groups = t.chase_groups * std::max(1, count);
+    if (!ctx->tune.bc_valu) ctx->tune.bc_valu = t.bc_valu;
+    if (!ctx->tune.sbr_back_block) ctx->tune.sbr_back_block = t.sbr_back_block;
+  }
+  if (nb == 0) nb = tnb > 0 ? tnb : b * std::max<long>(1, 1024 / b);
+  int s = arena_ensure(ctx, syevd_batched_workspace(count, n, b, nb, jobz, ctx->precision));
+  if (s == 0) s = syevd_batched(ctx, jobz, count, n, b, nb, A, lda, a_stride, w, w_stride, Z, ldz, z_stride);
   ctx->tune = user;
   return s;
 }

@@ -182,6 +182,22 @@ size_t evd_syevd_workspace_f32(long n, long b, long nb, char jobz);
  * its WY blocks, 16 KiB per 32 x 32 reflector block, in the arena). */
 size_t evd_syevd_workspace_ex(long n, long b, long nb, char jobz, int precision);
 
+/* ---- batched solves (no counterpart in the reference, which solves one
+ * matrix per process) ----
+ * count same-size problems in strided storage: A + q a_stride (overwritten),
+ * w + q w_stride, Z + q z_stride.  The bulge chase of all problems runs as one
+ * persistent launch and the BC back transformation as one launch per kernel;
+ * SBR, D&C and the SBR back transformation run per problem in order.  Results
+ * equal evd_syevd_f32 on each problem bit for bit. */
+int evd_syevd_batched_f32(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda,
+                          long a_stride, double *w, long w_stride, float *Z, long ldz, long z_stride);
+size_t evd_syevd_batched_workspace_f32(int count, long n, long b, long nb, char jobz, int precision);
+/* Bulge chasing of count bands (band + q band_stride, U + q u_stride,
+ * D + q d_stride, E + q e_stride) in one persistent launch: the batched form of
+ * evd_sb2st_f64. */
+int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride,
+                          float *U, long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride);
+
 /* ---- autotuning (no counterpart in the reference: its tile shapes are
  * compile-time template arguments and the myEVD nb argument) ----
  * Knob vector: {nb, chase_groups, bc_valu, sbr_back_block} (0 = default).
