@@ -12,6 +12,12 @@
 // deterministic with respect to co-scheduling (no floating-point atomics, a
 // fixed launch configuration per size), so a request gives the same bits alone
 // and in a mix.  Latency = completion - submission, per request.
+//
+// Same-size batching: when a slot takes a request of order n <= kBatchMaxN,
+// it also takes up to kBatchMax - 1 queued requests of the same (n, b, nb,
+// jobz), stages them into strided buffers and solves them with
+// evd_syevd_batched_f32 (one persistent chase launch and one BC back
+// transformation launch for the group; bitwise the same results).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -25,7 +31,11 @@
 
 namespace evd {
 size_t syevd_workspace(long n, long b, long nb, char jobz, int precision);
+size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision);
 }
+
+extern "C" int evd_syevd_batched_f32(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda,
+                                     long a_stride, double *w, long w_stride, float *Z, long ldz, long z_stride);
 
 namespace {
 
@@ -51,11 +61,17 @@ struct Done {
 
 }  // namespace
 
+constexpr long kBatchMaxN = 2048;  // orders that are batched
+constexpr int kBatchMax = 32;      // requests per batch
+
 struct evd_server {
   int device = 0;
   size_t budget = 0;
   std::vector<evd_ctx *> ctx;
-  std::vector<size_t> arena;  // bytes each slot's arena holds (grows only)
+  std::vector<size_t> arena;  // bytes each slot's arena (and batch staging) holds (grows only)
+  std::vector<char *> stage;  // per-slot staging for batched requests
+  std::vector<size_t> stage_bytes;
+  bool batching = true;
   std::vector<std::thread> workers;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -75,43 +91,118 @@ static size_t arena_total(const evd_server *s) {
   return t;
 }
 
+static size_t batch_stage_bytes(int count, long n) {
+  return evd::align256((size_t)count * n * n * sizeof(float)) * 2 + evd::align256((size_t)count * n * sizeof(double));
+}
+
+static bool same_class(const Request &a, const Request &b) {
+  return a.n == b.n && a.b == b.b && a.nb == b.nb && (a.Z != nullptr) == (b.Z != nullptr);
+}
+
+static void finish(evd_server *s, const std::vector<Request> &batch, int st) {
+  const clk::time_point t1 = clk::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const Request &r : batch) {
+      const double ms = std::chrono::duration<double, std::milli>(t1 - r.t_submit).count();
+      s->done[r.id] = Done{st, ms, t1};
+      s->lat.push_back(ms);
+    }
+    s->t_last = t1;
+    s->running -= (long)batch.size();
+  }
+  s->cv_done.notify_all();
+  s->cv_work.notify_all();
+}
+
+// A group of same-class requests through the batched solver: stage A into
+// strided storage, solve, copy w and Z back (all on the slot's stream).
+static int run_batch(evd_server *s, int slot, const std::vector<Request> &batch, size_t ws) {
+  evd_ctx *ctx = s->ctx[slot];
+  const int count = (int)batch.size();
+  const long n = batch[0].n;
+  const bool vec = batch[0].Z != nullptr;
+  const size_t need = batch_stage_bytes(count, n);
+  if (s->stage_bytes[slot] < need) {
+    if (s->stage[slot]) {
+      hipStreamSynchronize(ctx->stream);
+      hipFree(s->stage[slot]);
+      s->stage[slot] = nullptr;
+      s->stage_bytes[slot] = 0;
+    }
+    if (hipMalloc(&s->stage[slot], need) != hipSuccess) return EVD_E_NOMEM;
+    s->stage_bytes[slot] = need;
+  }
+  float *A = reinterpret_cast<float *>(s->stage[slot]);
+  float *Z = reinterpret_cast<float *>(s->stage[slot] + evd::align256((size_t)count * n * n * sizeof(float)));
+  double *w = reinterpret_cast<double *>(s->stage[slot] + 2 * evd::align256((size_t)count * n * n * sizeof(float)));
+  for (int q = 0; q < count; q++) EVD_TRY(evd::copy_f32(ctx, batch[q].A, batch[q].lda, A + (long)q * n * n, n, n, n));
+  EVD_TRY(evd_ctx_reserve(ctx, ws));
+  EVD_TRY(evd_syevd_batched_f32(ctx, vec ? 'V' : 'N', count, n, batch[0].b, batch[0].nb, A, n, n * n, w, n, Z, n,
+                                n * n));
+  for (int q = 0; q < count; q++) {
+    if (hipMemcpyAsync(batch[q].w, w + (long)q * n, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream) !=
+        hipSuccess)
+      return EVD_E_HIP;
+    if (vec) EVD_TRY(evd::copy_f32(ctx, Z + (long)q * n * n, n, batch[q].Z, batch[q].ldz, n, n));
+  }
+  return evd_ctx_synchronize(ctx);
+}
+
 static void worker_main(evd_server *s, int slot) {
   hipSetDevice(s->device);
   evd_ctx *ctx = s->ctx[slot];
   for (;;) {
-    Request r;
+    std::vector<Request> batch;
+    size_t bws = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       for (;;) {
         if (s->stop && s->queue.empty()) return;
         if (!s->queue.empty()) {
-          const Request &h = s->queue.front();
-          const size_t need = std::max(h.ws, s->arena[slot]);
-          if (arena_total(s) - s->arena[slot] + need <= s->budget) {
-            r = h;
+          const Request h = s->queue.front();
+          // same-class requests queued behind the head (FIFO among them)
+          std::vector<size_t> more;  // queue positions (erasing invalidates deque iterators)
+          if (s->batching && h.n <= kBatchMaxN && h.n > 2)
+            for (size_t i = 1; i < s->queue.size() && (int)more.size() + 1 < kBatchMax; i++)
+              if (same_class(s->queue[i], h)) more.push_back(i);
+          // largest group whose workspace fits the budget (a single request as before)
+          int cnt = 1 + (int)more.size();
+          size_t need = 0;
+          for (; cnt >= 1; cnt = (cnt > 2) ? cnt / 2 : cnt - 1) {
+            const size_t ws = (cnt == 1) ? h.ws
+                                         : evd::syevd_batched_workspace(cnt, h.n, h.b, h.nb, h.Z ? 'V' : 'N',
+                                                                        EVD_PREC_FP32) +
+                                               batch_stage_bytes(cnt, h.n);
+            need = std::max(ws, s->arena[slot]);
+            if (arena_total(s) - s->arena[slot] + need <= s->budget) {
+              bws = (cnt == 1) ? h.ws : ws - batch_stage_bytes(cnt, h.n);
+              break;
+            }
+          }
+          if (cnt >= 1) {
+            batch.push_back(h);
+            for (int i = 0; i + 1 < cnt; i++) batch.push_back(s->queue[more[i]]);
+            for (int i = cnt - 2; i >= 0; i--) s->queue.erase(s->queue.begin() + (long)more[i]);
             s->queue.pop_front();
             s->arena[slot] = need;
-            s->running++;
+            s->running += cnt;
             break;
           }
         }
         s->cv_work.wait(lk);
       }
     }
-    int st = evd_ctx_reserve(ctx, r.ws);
-    if (st == 0) st = evd_syevd_f32(ctx, r.Z ? 'V' : 'N', r.n, r.b, r.nb, r.A, r.lda, r.w, r.Z, r.ldz);
-    if (st == 0) st = evd_ctx_synchronize(ctx);
-    const clk::time_point t1 = clk::now();
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      const double ms = std::chrono::duration<double, std::milli>(t1 - r.t_submit).count();
-      s->done[r.id] = Done{st, ms, t1};
-      s->lat.push_back(ms);
-      s->t_last = t1;
-      s->running--;
+    int st;
+    if (batch.size() == 1) {
+      const Request &r = batch[0];
+      st = evd_ctx_reserve(ctx, r.ws);
+      if (st == 0) st = evd_syevd_f32(ctx, r.Z ? 'V' : 'N', r.n, r.b, r.nb, r.A, r.lda, r.w, r.Z, r.ldz);
+      if (st == 0) st = evd_ctx_synchronize(ctx);
+    } else {
+      st = run_batch(s, slot, batch, bws);
     }
-    s->cv_done.notify_all();
-    s->cv_work.notify_all();
+    finish(s, batch, st);
   }
 }
 
@@ -134,6 +225,8 @@ int evd_server_create(int device, int slots, size_t hbm_budget, evd_server **out
     }
     s->ctx.push_back(c);
     s->arena.push_back(0);
+    s->stage.push_back(nullptr);
+    s->stage_bytes.push_back(0);
   }
   for (int i = 0; i < slots; i++) s->workers.emplace_back(worker_main, s, i);
   *out = s;
@@ -148,6 +241,8 @@ int evd_server_destroy(evd_server *s) {
   }
   s->cv_work.notify_all();
   for (auto &t : s->workers) t.join();
+  for (char *p : s->stage)
+    if (p) hipFree(p);
   for (evd_ctx *c : s->ctx) evd_ctx_destroy(c);
   delete s;
   return 0;

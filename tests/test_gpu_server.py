@@ -82,3 +82,23 @@ def test_server_latency_statistics():
     srv.close()
     assert st["completed"] == 5 and st["solves_per_s"] > 0
     assert min(lat) > 0 and st["p50_ms"] <= st["p99_ms"] <= max(lat) + 1e-9
+
+
+def test_server_same_size_requests_match_single_solves():
+    """Requests of one size queued behind a running one are solved as a batch
+    (evd_syevd_batched_f32); each result must equal the single solve."""
+    import evd4sc2025_amd as E
+
+    reqs = [(384, "V", 20 + i) for i in range(6)] + [(384, "N", 30), (384, "N", 31)]
+    srv = E.Server(0, slots=1)
+    res = _solve_all(srv, reqs)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs)
+    ctx = E.Context(0)
+    for (n, jobz, seed), (w, Z) in zip(reqs, res):
+        r = E.syevd(ctx, to_dev(O.gen_symmetric(n, seed, 0)), b=32, nb=256, jobz=jobz)
+        assert np.array_equal(w, to_host(r.w)[:n]), seed
+        if jobz == "V":
+            assert np.array_equal(Z, to_host(r.Z)), seed
+    ctx.close()
