@@ -70,8 +70,9 @@ def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 
     """Serving form on a bounded sample: nreq requests, n drawn from
     {256: 50%, 1024: 30%, 4096: 15%, 16384: 5%}, 70% with eigenvectors,
     Gaussian-entry symmetric matrices resident in HBM.  Saturated throughput
-    with one slot (one-at-a-time solves) and with 4 concurrent slots, then the
-    same Poisson arrival schedule (70% of the one-slot capacity) on both."""
+    of one-at-a-time plain solves (one slot, no batching) and of the serving
+    engine (4 concurrent slots, same-size batching), then the same Poisson
+    arrival schedule (70% of the one-at-a-time capacity) on both."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -81,14 +82,14 @@ def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 
     pristine = [E.gen_symmetric(gctx, n, 1000 + i, 1) for i, n in enumerate(sizes)]
     torch.cuda.synchronize()
 
-    def run(slots, arrivals):
+    def run(slots, arrivals, batching):
         work = []
         for p in pristine:
             w = E.colmajor(p.shape[0], p.shape[0])
             w.copy_(p)
             work.append(w)
         torch.cuda.synchronize()
-        srv = E.Server(0, slots=slots, hbm_budget=budget)
+        srv = E.Server(0, slots=slots, hbm_budget=budget, batching=batching)
         t0 = time.perf_counter()
         for i in np.argsort(arrivals, kind="stable"):
             delay = t0 + arrivals[i] - time.perf_counter()
@@ -103,16 +104,17 @@ def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 
         return {k: st[k] for k in ("solves_per_s", "p50_ms", "p99_ms")} | {"arena_gib": st["arena_bytes"] / 2**30}
 
     zero = np.zeros(nreq)
-    sat1 = run(1, zero)
-    sat4 = run(4, zero)
+    sat1 = run(1, zero, False)
+    sat4 = run(4, zero, True)
     rate = 0.7 * sat1["solves_per_s"]
     arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-    poi1 = run(1, arr)
-    poi4 = run(4, arr)
+    poi1 = run(1, arr, False)
+    poi4 = run(4, arr, True)
     return {"requests": nreq, "mix": "n in {256: 50%, 1024: 30%, 4096: 15%, 16384: 5%}, 70% jobz=V, Gaussian "
                                      f"entries, seed {seed}, plain FP32, HBM budget {budget >> 30} GiB",
-            "saturated": {"slots1": sat1, "slots4": sat4, "throughput_ratio": sat4["solves_per_s"] / sat1["solves_per_s"]},
-            "poisson": {"rate_per_s": rate, "slots1": poi1, "slots4": poi4}}
+            "saturated": {"one_at_a_time": sat1, "engine_4slots_batching": sat4,
+                          "throughput_ratio": sat4["solves_per_s"] / sat1["solves_per_s"]},
+            "poisson": {"rate_per_s": rate, "one_at_a_time": poi1, "engine_4slots_batching": poi4}}
 
 
 def main():

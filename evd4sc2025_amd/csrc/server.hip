@@ -233,6 +233,13 @@ int evd_server_create(int device, int slots, size_t hbm_budget, evd_server **out
   return 0;
 }
 
+int evd_server_set_batching(evd_server *s, int on) {
+  if (!s) return -1;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->batching = on != 0;
+  return 0;
+}
+
 int evd_server_destroy(evd_server *s) {
   if (!s) return 0;
   {

@@ -262,7 +262,11 @@ int evd_gen_spectrum_f32(evd_ctx *ctx, long n, unsigned long long seed, const do
  * what else runs: a request returns the same bits alone and in a mix. */
 typedef struct evd_server evd_server;
 int evd_server_create(int device, int slots, size_t hbm_budget, evd_server **out);
-int evd_server_destroy(evd_server *srv); /* finishes queued requests first */
+int evd_server_destroy(evd_server *srv);
+/* Same-size batching of queued requests of order <= 2048 (default on): a slot
+ * that takes a request also takes up to 31 queued requests of the same
+ * (n, b, nb, jobz) and solves them as one evd_syevd_batched_f32 call. */
+int evd_server_set_batching(evd_server *srv, int on); /* finishes queued requests first */
 /* Non-blocking; *id receives the request id.  EVD_E_NOMEM if the request's
  * workspace alone exceeds the budget. */
 int evd_server_submit(evd_server *srv, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz,
