@@ -22,7 +22,7 @@ __all__ = [
     "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "hou_tsqr_panel", "tc_ozimmu_syr2k", "band_pack",
     "sb2st", "stedc", "computerWYFromSlide", "computerQFromWY", "my_BC_back_trans_v8_10_noBandU", "bc_back",
     "sbr_back", "checkOrthogonality", "backward_error", "sgemm", "dgemm", "gen_symmetric", "gen_spectrum",
-    "STAGES", "autotune", "syevd_batched", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
+    "STAGES", "autotune", "syevd_batched", "SolveGraph", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
 ]
 
 # GEMM precision of SBR and the back transformation (evd_ctx_set_precision)
@@ -369,6 +369,59 @@ def syevd(ctx: Context, A, b: int = 32, nb: int = 1024, jobz: str = "V", overwri
     ctx.call("evd_syevd_f32", _ch(jobz), n, b, nb, C.c_void_p(pa), lda, C.c_void_p(w.data_ptr()), C.c_void_p(pz),
              ldz)
     return EigResult(w[:n], Z, ctx.stage_ms())
+
+
+class SolveGraph:
+    """A whole solve of order n captured once as a hipGraph (evd_graph_*).
+
+    The graph owns fixed device buffers: `A` (column-major n x n FP32, lower
+    triangle read and overwritten), `w` (float64) and, for jobz="V", `Z`.
+    `solve(M)` copies M into A on the context's stream, replays the graph and
+    waits; the results are bitwise those of `syevd` on M.  Creating the graph
+    solves the matrix `A0` once eagerly (its results are in w/Z afterwards).
+    While the graph lives the context's arena cannot grow and the context
+    cannot be closed: call `close()` first.
+    """
+
+    def __init__(self, ctx: Context, A0, b: int = 32, nb: int = 1024, jobz: str = "V"):
+        torch = _torch()
+        self.ctx = ctx
+        self.n = n = A0.shape[0]
+        self.jobz = jobz
+        self.A = colmajor(n, n, device=A0.device)
+        self.A.copy_(A0)
+        self.w = torch.empty(n, dtype=torch.float64, device=A0.device)
+        self.Z = colmajor(n, n, device=A0.device) if jobz == "V" else None
+        pa, lda = _cm(self.A)
+        pz, ldz = (_cm(self.Z) if self.Z is not None else (0, n))
+        h = C.c_void_p()
+        ctx.call("evd_graph_create", _ch(jobz), n, b, nb, C.c_void_p(pa), lda, C.c_void_p(self.w.data_ptr()),
+                 C.c_void_p(pz), ldz, C.byref(h))
+        self.h = h
+
+    def launch(self):
+        self.ctx.check("evd_graph_launch", self.ctx._lib.evd_graph_launch(self.h))
+
+    def wait(self):
+        self.ctx.check("evd_graph_wait", self.ctx._lib.evd_graph_wait(self.h))
+
+    def solve(self, M):
+        """Eigen-decomposition of M (n x n, lower triangle read) by one replay."""
+        self.A.copy_(M)
+        self.launch()
+        self.wait()
+        return EigResult(self.w, self.Z, {})
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx._lib.evd_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def syevd_workspace(n: int, b: int, nb: int, jobz: str = "V") -> int:

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "../../include/evd4sc.h"
@@ -24,6 +25,17 @@ struct Tuning {
   int chase_groups = 0;    // persistent workgroups of the bulge chase (0 = auto)
   int sbr_back_block = 0;  // WY block width of the SBR back transformation (0 = 1024)
   int bc_valu = 0;         // 1: BC back transformation on the FP32 VALU kernel only
+};
+
+// Merge plan of the tridiagonal divide and conquer for one order n (host
+// level table + device copy of the merge descriptors, split flags and tile
+// offsets), kept per context.
+struct DcPlan {
+  int nmerge = 0, H = 0;
+  std::vector<int> level_off;    // merges of height h: [level_off[h], level_off[h + 1])
+  std::vector<int> level_maxN;   // largest merge order of each height
+  std::vector<int> level_tiles;  // grouped-GEMM tile budget of each height
+  void *dev = nullptr;
 };
 
 }  // namespace evd
@@ -45,11 +57,14 @@ struct evd_ctx {
   unsigned *ctrl = nullptr;  // kCtrlWords words, zeroed at creation
   size_t ctrl_words = 0;
 
-  unsigned epoch = 0;        // launch counter for tag-based exchanges
   unsigned *tags = nullptr;  // kTagWords persistent exchange tags, zeroed at creation
   evd::Tuning tune;
   bool tune_cache = true;  // fill knobs left at 0 from the process-wide tuning cache
   int precision = EVD_PREC_FP32;
+
+  std::map<long, evd::DcPlan> dc_plans;  // tridiagonal merge plans by order
+  bool capturing = false;                 // inside hipGraph capture: no host round trips
+  int graphs = 0;                         // live captured graphs: the arena may not move
 
   bool timing = false;
   hipEvent_t ev[EVD_STAGE_COUNT + 1] = {};
@@ -64,6 +79,7 @@ namespace evd {
 // control block layout (words)
 enum : int {
   kCtrlError = 0,      // device-side error word (EVD_E_*), sticky until read
+  kCtrlEpoch = 8,      // launch epoch of the panel factorisation's exchange
   kCtrlBarrier = 16,   // grid barrier: [count, generation] pairs, 64 slots
   kCtrlWordsTotal = 16 + 2 * 64,
   kTagWords = 4096,
@@ -151,6 +167,9 @@ int max_coresident_blocks(evd_ctx *ctx, const void *kernel, int block, size_t dy
 
 // error word readback (synchronises the stream)
 int check_device_error(evd_ctx *ctx, const char *where);
+
+// evd_syevd_f32 after its argument checks (syevd.hip)
+int syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz);
 
 }  // namespace evd
 

@@ -23,6 +23,10 @@ int arena_ensure(evd_ctx *ctx, size_t bytes) {
   if (ctx->arena_top != 0)
     return set_error(ctx, EVD_E_NOMEM, "arena too small inside a call (need %zu, have %zu); reserve first", need,
                      ctx->arena_bytes);
+  if (ctx->graphs > 0)
+    return set_error(ctx, EVD_E_UNSUPPORTED,
+                     "arena of %zu bytes cannot grow to %zu while captured graphs use it; destroy them first",
+                     ctx->arena_bytes, need);
   if (ctx->budget && need > ctx->budget)
     return set_error(ctx, EVD_E_NOMEM, "arena of %zu bytes exceeds the budget of %zu bytes", need, ctx->budget);
   if (ctx->arena) {
@@ -209,7 +213,10 @@ int evd_ctx_create(int device, void *stream, evd_ctx **out) {
 
 int evd_ctx_destroy(evd_ctx *ctx) {
   if (!ctx) return 0;
+  if (ctx->graphs > 0) return set_error(ctx, EVD_E_UNSUPPORTED, "%d captured graphs still use this context", ctx->graphs);
   hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->dc_plans)
+    if (kv.second.dev) hipFree(kv.second.dev);
   if (ctx->arena) hipFree(ctx->arena);
   if (ctx->ctrl) hipFree(ctx->ctrl);
   if (ctx->tags) hipFree(ctx->tags);

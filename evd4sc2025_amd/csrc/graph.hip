@@ -1,0 +1,91 @@
+// Captured solves: the whole eigensolver of one (n, b, nb, jobz) on fixed
+// buffers recorded once as a hipGraph and replayed.  The reference launches
+// every kernel from the host on every call (src/EVD/myEVD.cu:251-706); for
+// small orders that launch stream is the cost, and a replay submits it as one
+// unit.  The solve makes no host round trip between its kernels (merge plans
+// and occupancy are resolved by the eager solve that precedes the capture,
+// launch epochs of the panel exchange are kept on the device), so the
+// captured work is the eager call's work exactly.
+#include "common.hpp"
+
+struct evd_graph {
+  evd_ctx *ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+using namespace evd;
+
+extern "C" int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w,
+                                float *Z, long ldz, evd_graph **out) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, out != nullptr, 11);
+  EVD_ARG(ctx, n >= 1, 3);
+  *out = nullptr;
+  // eager solve: checks the arguments, sizes the arena, builds the merge plan
+  EVD_TRY(evd_syevd_f32(ctx, jobz, n, b, nb, A, lda, w, Z, ldz));
+
+  // Capture on a private non-blocking stream (the context's stream may be the
+  // null stream, which cannot be captured); replays go to the context's
+  // stream.
+  hipStream_t cap = nullptr;
+  EVD_HIP(ctx, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipStream_t saved_stream = ctx->stream;
+  const bool saved_timing = ctx->timing;
+  ctx->stream = cap;
+  ctx->timing = false;
+  ctx->capturing = true;
+  int status = 0;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    status = set_error(ctx, EVD_E_HIP, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+  } else {
+    status = syevd_call(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+    e = hipStreamEndCapture(cap, &graph);
+    if (status == 0 && e != hipSuccess)
+      status = set_error(ctx, EVD_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  }
+  ctx->capturing = false;
+  ctx->timing = saved_timing;
+  ctx->stream = saved_stream;
+  hipStreamDestroy(cap);
+  hipGetLastError();  // a failed capture leaves its error here; it is reported above
+  hipGraphExec_t exec = nullptr;
+  if (status == 0) {
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) status = set_error(ctx, EVD_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  if (status != 0) {
+    if (graph) hipGraphDestroy(graph);
+    return status;
+  }
+  evd_graph *g = new evd_graph;
+  g->ctx = ctx;
+  g->graph = graph;
+  g->exec = exec;
+  ctx->graphs++;
+  *out = g;
+  return 0;
+}
+
+extern "C" int evd_graph_launch(evd_graph *g) {
+  if (!g) return -1;
+  EVD_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
+  return 0;
+}
+
+extern "C" int evd_graph_wait(evd_graph *g) {
+  if (!g) return -1;
+  return check_device_error(g->ctx, "evd_graph_launch");
+}
+
+extern "C" int evd_graph_destroy(evd_graph *g) {
+  if (!g) return 0;
+  hipStreamSynchronize(g->ctx->stream);
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  g->ctx->graphs--;
+  delete g;
+  return 0;
+}

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
                                                        long ldw, float *__restrict__ R, long ldr,
                                                        float *__restrict__ Aout, long ldo, double *__restrict__ part,
                                                        double *__restrict__ tau_out, unsigned *__restrict__ tags,
-                                                       unsigned epoch, unsigned *__restrict__ err) {
+                                                       unsigned *__restrict__ epoch_word, unsigned *__restrict__ err) {
   constexpr int CH = NB < 32 ? NB : 32;  // slots per in-wave reduction
   constexpr int PASSG = 32;              // group records gathered per pass
   static_assert(NB <= 256, "the pivot rows must live in group 0");
@@ -101,6 +101,13 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
     (&gram[0][0])[i] = 0.0;
     (&Tt[0][0])[i] = 0.0;
   }
+  // Launch epoch: one more than the last launch's, read from a device word
+  // (so a launch replayed from a captured graph still gets a fresh epoch and
+  // stale tags or sentinels of earlier launches never match).  Every group
+  // reads the word before it publishes its start tag; group 0 advances the
+  // word only after it has seen every start tag.
+  unsigned epoch = (ld_dev(epoch_word) + 1u) & 0x1FFFFFFu;
+  if (epoch == 0) epoch = 1;
   if (G > 1) {
     // sentinels of steps 0 and 1 in this group's records, then a one-time
     // start barrier so nobody polls a slot before its owner marked it
@@ -127,6 +134,7 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
       }
     }
     __syncthreads();
+    if (g == 0 && tid == 0) st_dev(epoch_word, epoch);
     if (!s_ok) return;
   }
 
@@ -354,12 +362,11 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
   if (!part) return EVD_E_NOMEM;
   if (G > kTagWords) return set_error(ctx, EVD_E_UNSUPPORTED, "panel needs %d exchange tags", G);
   unsigned *tags = ctx->tags;
-  // tags carry a launch epoch, so stale tags of earlier launches never match
-  unsigned epoch = (++ctx->epoch) & 0x1FFFFFFu;
-  if (epoch == 0) epoch = (++ctx->epoch) & 0x1FFFFFFu;
+  unsigned *epoch_word = ctx->ctrl + kCtrlEpoch;
   unsigned *err = ctx->ctrl + kCtrlError;
   int mi = (int)m, ni = (int)nc;
-  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &tau_out, &tags, &epoch, &err};
+  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R,
+                  &ldr, &Aout, &ldo, &part, &tau_out, &tags, &epoch_word, &err};
   // one workgroup is trivially co-resident: a regular launch, which (unlike a
   // cooperative one) may overlap with the work of other streams
   if (G == 1)

@@ -68,6 +68,16 @@ __global__ void dc_tear_kernel(long n, const int *__restrict__ split, const doub
   }
 }
 
+// |e| and sign(e) at every split, read on the device so no host round trip
+// sits between the tridiagonal stage's producer and the merges.
+__global__ void dc_rho_kernel(int nmerge, MergeDesc *__restrict__ md, const double *__restrict__ E) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nmerge; i += gridDim.x * blockDim.x) {
+    const double e = E[md[i].start + md[i].n1 - 1];
+    md[i].rho = fabs(e);
+    md[i].sgn = (e >= 0.0) ? 1.0 : -1.0;
+  }
+}
+
 __global__ void dc_sort_kernel(const MergeDesc *__restrict__ md, const double *__restrict__ D,
                                const double *__restrict__ Z, long ldz, DCScratch sc) {
   const MergeDesc m = md[blockIdx.y];
@@ -640,6 +650,47 @@ __global__ void dc_copy_deflated_kernel(const MergeDesc *__restrict__ md, const 
   }
 }
 
+// Grouped-GEMM problems of one tree level, built on the device from the
+// deflation results: merge i gives problem 2i (top rows: Qc[:, 0:K1+K2]) and
+// 2i+1 (bottom rows: Qc[:, K1:K]).  Tile offsets are fixed per level from the
+// bound K <= N; tiles past a problem's actual K columns exit at once, and a
+// merge with K = 0 gets m = 0.
+__global__ void dc_problems_kernel(int nm, const MergeDesc *__restrict__ md, const MergeOut *__restrict__ mo,
+                                   const int *__restrict__ tile_begin, DCScratch sc, long ldq, double *Z, long ldz,
+                                   DGemmProblem *__restrict__ probs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  const MergeDesc m = md[i];
+  const MergeOut o = mo[i];
+  const int st = m.start;
+  DGemmProblem top;
+  top.m = o.K > 0 ? m.n1 : 0;
+  top.n = o.K;
+  top.k = o.K1 + o.K2;
+  top.tile_begin = tile_begin[2 * i];
+  top.A = sc.Qc + st + (long)st * ldq;
+  top.lda = ldq;
+  top.B = sc.Uc + st + (long)st * ldq;
+  top.ldb = ldq;
+  top.C = Z + st;
+  top.ldc = ldz;
+  top.colmap = sc.colmap + st;
+  probs[2 * i] = top;
+  DGemmProblem bot;
+  bot.m = o.K > 0 ? m.n2 : 0;
+  bot.n = o.K;
+  bot.k = o.K2 + o.K3;
+  bot.tile_begin = tile_begin[2 * i + 1];
+  bot.A = sc.Qc + (st + m.n1) + (long)(st + o.K1) * ldq;
+  bot.lda = ldq;
+  bot.B = sc.Uc + (st + o.K1) + (long)st * ldq;
+  bot.ldb = ldq;
+  bot.C = Z + st + m.n1;
+  bot.ldc = ldz;
+  bot.colmap = sc.colmap + st;
+  probs[2 * i + 1] = bot;
+}
+
 struct TreeNode {
   int start, size, left, right, height;
 };
@@ -663,9 +714,79 @@ size_t stedc_workspace(long n) {
   b += align256((size_t)n * n * sizeof(double)) * 2;  // Qc, Uc
   b += align256((size_t)n * sizeof(double)) * 12;
   b += align256((size_t)n * sizeof(int)) * 12;
-  b += align256((size_t)n * (sizeof(MergeDesc) + sizeof(MergeOut)));
+  b += align256((size_t)n * sizeof(MergeOut));
   b += align256((size_t)2 * n * sizeof(DGemmProblem));
   return b + (1 << 16);
+}
+
+// Merge plan of order n, built once per context: the tree (halving splits
+// down to single rows), merges grouped by height, the split flags and the
+// per-level tile offsets of the grouped GEMMs, copied to device memory that
+// lives as long as the context.  Only the split values (rho, sign) change per
+// solve; dc_rho_kernel writes them into the device descriptors.
+int dc_plan(evd_ctx *ctx, long n, const DcPlan **out) {
+  auto it = ctx->dc_plans.find(n);
+  if (it != ctx->dc_plans.end()) {
+    *out = &it->second;
+    return 0;
+  }
+  if (ctx->capturing) return set_error(ctx, EVD_E_UNSUPPORTED, "D&C plan for n = %ld missing during capture", n);
+  std::vector<TreeNode> nodes;
+  nodes.reserve(2 * n);
+  build_tree(nodes, 0, (int)n);
+  DcPlan plan;
+  plan.H = nodes[0].height;
+  const int H = plan.H;
+  std::vector<std::vector<int>> by_height(H + 1);
+  for (int i = 0; i < (int)nodes.size(); i++)
+    if (nodes[i].size > 1) by_height[nodes[i].height].push_back(i);
+  std::vector<MergeDesc> all;
+  std::vector<int> tiles;
+  plan.level_off.assign(H + 2, 0);
+  plan.level_maxN.assign(H + 1, 0);
+  plan.level_tiles.assign(H + 1, 0);
+  auto ceil_div = [](int a, int b) { return (a + b - 1) / b; };
+  for (int h = 1; h <= H; h++) {
+    plan.level_off[h] = (int)all.size();
+    int t = 0;
+    for (int id : by_height[h]) {
+      const TreeNode &nd = nodes[id];
+      MergeDesc m;
+      m.start = nd.start;
+      m.n1 = nodes[nd.left].size;
+      m.n2 = nodes[nd.right].size;
+      m.pad = 0;
+      m.rho = 0.0;
+      m.sgn = 1.0;
+      all.push_back(m);
+      const int N = m.n1 + m.n2, tn = ceil_div(N, kDGroupTile);
+      plan.level_maxN[h] = std::max(plan.level_maxN[h], N);
+      tiles.push_back(t);
+      t += ceil_div(m.n1, kDGroupTile) * tn;
+      tiles.push_back(t);
+      t += ceil_div(m.n2, kDGroupTile) * tn;
+    }
+    plan.level_tiles[h] = t;
+  }
+  plan.level_off[H + 1] = (int)all.size();
+  plan.nmerge = (int)all.size();
+  std::vector<int> split(n, 0);
+  for (const MergeDesc &m : all) split[m.start + m.n1] = 1;
+  const size_t b_md = align256(sizeof(MergeDesc) * all.size()), b_split = align256(sizeof(int) * n),
+               b_tiles = align256(sizeof(int) * tiles.size());
+  std::vector<char> host(b_md + b_split + b_tiles, 0);
+  memcpy(host.data(), all.data(), sizeof(MergeDesc) * all.size());
+  memcpy(host.data() + b_md, split.data(), sizeof(int) * n);
+  memcpy(host.data() + b_md + b_split, tiles.data(), sizeof(int) * tiles.size());
+  EVD_HIP(ctx, hipMalloc(&plan.dev, host.size()));
+  hipError_t e = hipMemcpy(plan.dev, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(plan.dev);
+    return set_error(ctx, EVD_E_HIP, "D&C plan upload failed: %s", hipGetErrorString(e));
+  }
+  auto ins = ctx->dc_plans.emplace(n, std::move(plan));
+  *out = &ins.first->second;
+  return 0;
 }
 
 int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
@@ -685,139 +806,62 @@ int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
   for (auto p : ibufs) *p = arena_alloc_n<int>(ctx, (size_t)n);
   if (!sc.Qc || !sc.Uc || !sc.zhat || !sc.dmap) return EVD_E_NOMEM;
 
-  // ---- host tree, merges grouped by height
-  std::vector<TreeNode> nodes;
-  nodes.reserve(2 * n);
-  build_tree(nodes, 0, (int)n);
-  const int H = nodes[0].height;
-  std::vector<std::vector<int>> by_height(H + 1);
-  for (int i = 0; i < (int)nodes.size(); i++)
-    if (nodes[i].size > 1) by_height[nodes[i].height].push_back(i);
-  std::vector<double> hE(n - 1);
-  EVD_HIP(ctx, hipMemcpyAsync(hE.data(), E, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, ctx->stream));
-  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<MergeDesc> all;
-  std::vector<int> level_off(H + 2, 0);
-  for (int h = 1; h <= H; h++) {
-    level_off[h] = (int)all.size();
-    for (int id : by_height[h]) {
-      const TreeNode &t = nodes[id];
-      MergeDesc m;
-      m.start = t.start;
-      m.n1 = nodes[t.left].size;
-      m.n2 = nodes[t.right].size;
-      m.pad = 0;
-      double e = hE[t.start + m.n1 - 1];
-      m.rho = fabs(e);
-      m.sgn = (e >= 0.0) ? 1.0 : -1.0;
-      all.push_back(m);
-    }
-  }
-  level_off[H + 1] = (int)all.size();
-  const int nmerge = (int)all.size();
-  MergeDesc *dmd = static_cast<MergeDesc *>(arena_alloc(ctx, sizeof(MergeDesc) * nmerge));
+  const DcPlan *plan = nullptr;
+  EVD_TRY(dc_plan(ctx, n, &plan));
+  const int nmerge = plan->nmerge, H = plan->H;
+  char *pdev = static_cast<char *>(plan->dev);
+  MergeDesc *dmd = reinterpret_cast<MergeDesc *>(pdev);
+  const int *dsplit = reinterpret_cast<const int *>(pdev + align256(sizeof(MergeDesc) * nmerge));
+  const int *dtiles = reinterpret_cast<const int *>(pdev + align256(sizeof(MergeDesc) * nmerge) +
+                                                    align256(sizeof(int) * n));
   MergeOut *dmo = static_cast<MergeOut *>(arena_alloc(ctx, sizeof(MergeOut) * nmerge));
   DGemmProblem *dprob = static_cast<DGemmProblem *>(arena_alloc(ctx, sizeof(DGemmProblem) * 2 * nmerge));
-  if (!dmd || !dmo || !dprob) return EVD_E_NOMEM;
-  EVD_HIP(ctx, hipMemcpyAsync(dmd, all.data(), sizeof(MergeDesc) * nmerge, hipMemcpyHostToDevice, ctx->stream));
+  if (!dmo || !dprob) return EVD_E_NOMEM;
   unsigned *err = ctx->ctrl + kCtrlError;
 
-  // tearing of every split, then Q = I
-  std::vector<int> hsplit(n, 0);
-  for (const MergeDesc &m : all) hsplit[m.start + m.n1] = 1;
-  int *dsplit = sc.colmap;  // scratch until the first merge writes column maps
-  EVD_HIP(ctx, hipMemcpyAsync(dsplit, hsplit.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  // split values and signs, tearing of every split, then Q = I
+  dc_rho_kernel<<<(unsigned)std::min((nmerge + 255) / 256, 1024), 256, 0, ctx->stream>>>(nmerge, dmd, E);
+  EVD_LAUNCHED(ctx);
   dc_tear_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, dsplit, E, D);
   EVD_LAUNCHED(ctx);
-  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));  // hsplit is pageable host memory
   EVD_TRY(set_identity_f64(ctx, Z, n, ldz));
 
-  std::vector<MergeOut> hmo(nmerge);
-  std::vector<DGemmProblem> probs;
+  // Merges level by level (by height).  Launch shapes come from the largest
+  // merge of the level; every kernel bounds its work by the deflation
+  // results it reads on the device, so no level waits for the host.
   for (int h = 1; h <= H; h++) {
-    const int off = level_off[h], nm = level_off[h + 1] - off;
+    const int off = plan->level_off[h], nm = plan->level_off[h + 1] - off;
     if (nm == 0) continue;
     const MergeDesc *lmd = dmd + off;
     MergeOut *lmo = dmo + off;
-    int maxN = 0;
-    for (int i = 0; i < nm; i++) maxN = std::max(maxN, all[off + i].n1 + all[off + i].n2);
+    const int maxN = plan->level_maxN[h];
     const unsigned gx = (unsigned)std::min(64, (maxN + 255) / 256);
     dc_sort_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, D, Z, ldz, sc);
     EVD_LAUNCHED(ctx);
     dc_deflate_kernel<<<nm, 256, 0, ctx->stream>>>(lmd, lmo, sc);
     EVD_LAUNCHED(ctx);
-    EVD_HIP(ctx, hipMemcpyAsync(hmo.data() + off, lmo, sizeof(MergeOut) * nm, hipMemcpyDeviceToHost, ctx->stream));
-    EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int maxK = 0;
-    long maxDefl = 0;
-    for (int i = 0; i < nm; i++) {
-      maxK = std::max(maxK, hmo[off + i].K);
-      maxDefl = std::max<long>(maxDefl, (long)hmo[off + i].ndefl * (all[off + i].n1 + all[off + i].n2));
-    }
     dc_rotate_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, lmo, Z, ldz, sc);
     EVD_LAUNCHED(ctx);
     dc_gather_kernel<<<dim3((unsigned)((maxN + 255) / 256), (unsigned)((maxN + 63) / 64), nm), 256, 0, ctx->stream>>>(
         lmd, lmo, Z, ldz, sc, ldq);
     EVD_LAUNCHED(ctx);
-    if (maxK > 0) {
-      const unsigned gk = (unsigned)std::min(256, (maxK + 255) / 256);
-      const unsigned gw = (unsigned)std::min(16384, (maxK + 3) / 4);  // 4 roots (waves) per workgroup
-      dc_secular_kernel<<<dim3(gw, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, err);
-      EVD_LAUNCHED(ctx);
-      dc_loewner_kernel<<<dim3(gk, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc);
-      EVD_LAUNCHED(ctx);
-      dc_vectors_kernel<<<dim3(maxK, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq);
-      EVD_LAUNCHED(ctx);
-    }
+    const unsigned gk = (unsigned)std::min(256, (maxN + 255) / 256);
+    const unsigned gw = (unsigned)std::min(16384, (maxN + 3) / 4);  // 4 roots (waves) per workgroup
+    dc_secular_kernel<<<dim3(gw, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, err);
+    EVD_LAUNCHED(ctx);
+    dc_loewner_kernel<<<dim3(gk, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc);
+    EVD_LAUNCHED(ctx);
+    dc_vectors_kernel<<<dim3(maxN, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq);
+    EVD_LAUNCHED(ctx);
     dc_order_kernel<<<dim3(gx, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, D);
     EVD_LAUNCHED(ctx);
-    // grouped GEMMs: top rows use Qc[:, 0:K1+K2], bottom rows Qc[:, K1:K]
-    probs.clear();
-    int tiles = 0;
-    for (int i = 0; i < nm; i++) {
-      const MergeDesc &m = all[off + i];
-      const MergeOut &o = hmo[off + i];
-      if (o.K == 0) continue;
-      const int st = m.start;
-      DGemmProblem top;
-      top.m = m.n1;
-      top.n = o.K;
-      top.k = o.K1 + o.K2;
-      top.A = sc.Qc + st + (long)st * ldq;
-      top.lda = ldq;
-      top.B = sc.Uc + st + (long)st * ldq;
-      top.ldb = ldq;
-      top.C = Z + st;
-      top.ldc = ldz;
-      top.colmap = sc.colmap + st;
-      top.tile_begin = tiles;
-      tiles += ((top.m + kDGroupTile - 1) / kDGroupTile) * ((top.n + kDGroupTile - 1) / kDGroupTile);
-      probs.push_back(top);
-      DGemmProblem bot;
-      bot.m = m.n2;
-      bot.n = o.K;
-      bot.k = o.K2 + o.K3;
-      bot.A = sc.Qc + (st + m.n1) + (long)(st + o.K1) * ldq;
-      bot.lda = ldq;
-      bot.B = sc.Uc + (st + o.K1) + (long)st * ldq;
-      bot.ldb = ldq;
-      bot.C = Z + st + m.n1;
-      bot.ldc = ldz;
-      bot.colmap = sc.colmap + st;
-      bot.tile_begin = tiles;
-      tiles += ((bot.m + kDGroupTile - 1) / kDGroupTile) * ((bot.n + kDGroupTile - 1) / kDGroupTile);
-      probs.push_back(bot);
-    }
-    if (!probs.empty()) {
-      EVD_HIP(ctx, hipMemcpyAsync(dprob, probs.data(), sizeof(DGemmProblem) * probs.size(), hipMemcpyHostToDevice,
-                                  ctx->stream));
-      EVD_TRY(dgemm_grouped(ctx, dprob, (int)probs.size(), tiles));
-    }
-    if (maxDefl > 0) {
-      const unsigned gd = (unsigned)std::min<long>((maxDefl + 255) / 256, 1024);
-      dc_copy_deflated_kernel<<<dim3(gd, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq, Z, ldz);
-      EVD_LAUNCHED(ctx);
-    }
+    dc_problems_kernel<<<(unsigned)((nm + 255) / 256), 256, 0, ctx->stream>>>(nm, lmd, lmo, dtiles + 2 * off, sc, ldq,
+                                                                             Z, ldz, dprob);
+    EVD_LAUNCHED(ctx);
+    EVD_TRY(dgemm_grouped(ctx, dprob, 2 * nm, plan->level_tiles[h]));
+    const unsigned gd = (unsigned)std::min<long>(((long)maxN * maxN + 255) / 256, 1024);
+    dc_copy_deflated_kernel<<<dim3(gd, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq, Z, ldz);
+    EVD_LAUNCHED(ctx);
   }
   return 0;
 }

@@ -41,8 +41,44 @@ static long wy_cols(long n) { return (n + 1023) / 1024 * 1024; }
 // ------------------------------------------------------------- bisection
 // i-th smallest eigenvalue of the symmetric tridiagonal (D, E) by Sturm
 // counts (negative pivots of the LDL^T factorisation of T - x I).
-__global__ void sturm_bisect_kernel(int n, const double *__restrict__ D, const double *__restrict__ E, double gl,
-                                    double gu, double pivmin, double *__restrict__ w) {
+// Gershgorin interval [gl, gu] (widened by a few ulps of its span) and the
+// pivot floor of the Sturm counts, bnd = {gl, gu, pivmin}: one workgroup,
+// min/max reductions (exact in any order).
+__global__ __launch_bounds__(256) void gershgorin_kernel(int n, const double *__restrict__ D,
+                                                         const double *__restrict__ E, double *__restrict__ bnd) {
+  __shared__ double s_lo[256], s_hi[256], s_e2[256];
+  double lo = DBL_MAX, hi = -DBL_MAX, e2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double r = (i > 0 ? fabs(E[i - 1]) : 0.0) + (i < n - 1 ? fabs(E[i]) : 0.0);
+    lo = fmin(lo, D[i] - r);
+    hi = fmax(hi, D[i] + r);
+    if (i < n - 1) e2 = fmax(e2, E[i] * E[i]);
+  }
+  s_lo[threadIdx.x] = lo;
+  s_hi[threadIdx.x] = hi;
+  s_e2[threadIdx.x] = e2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s_lo[threadIdx.x] = fmin(s_lo[threadIdx.x], s_lo[threadIdx.x + o]);
+      s_hi[threadIdx.x] = fmax(s_hi[threadIdx.x], s_hi[threadIdx.x + o]);
+      s_e2[threadIdx.x] = fmax(s_e2[threadIdx.x], s_e2[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double gl = s_lo[0], gu = s_hi[0];
+    const double span = fmax(gu - gl, DBL_MIN);
+    const double pad = __dadd_rn(__dmul_rn(2.0 * DBL_EPSILON, span), DBL_MIN);
+    bnd[0] = gl - pad;
+    bnd[1] = gu + pad;
+    bnd[2] = fmax(DBL_MIN, __dmul_rn(DBL_MIN, s_e2[0]));
+  }
+}
+
+__global__ void sturm_bisect_kernel(int n, const double *__restrict__ D, const double *__restrict__ E,
+                                    const double *__restrict__ bnd, double *__restrict__ w) {
+  const double gl = bnd[0], gu = bnd[1], pivmin = bnd[2];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double lo = gl, hi = gu;
     for (int it = 0; it < 128; it++) {
@@ -68,22 +104,12 @@ __global__ void sturm_bisect_kernel(int n, const double *__restrict__ D, const d
 }
 
 static int sterf_bisect(evd_ctx *ctx, long n, const double *D, const double *E, double *w) {
-  std::vector<double> hD(n), hE(std::max<long>(n - 1, 1));
-  EVD_HIP(ctx, hipMemcpyAsync(hD.data(), D, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-  if (n > 1) EVD_HIP(ctx, hipMemcpyAsync(hE.data(), E, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, ctx->stream));
-  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
-  for (long i = 0; i < n; i++) {
-    double r = (i > 0 ? fabs(hE[i - 1]) : 0.0) + (i < n - 1 ? fabs(hE[i]) : 0.0);
-    gl = std::min(gl, hD[i] - r);
-    gu = std::max(gu, hD[i] + r);
-    if (i < n - 1) emax2 = std::max(emax2, hE[i] * hE[i]);
-  }
-  const double span = std::max(gu - gl, DBL_MIN);
-  gl -= 2.0 * DBL_EPSILON * span + DBL_MIN;
-  gu += 2.0 * DBL_EPSILON * span + DBL_MIN;
-  const double pivmin = std::max(DBL_MIN, DBL_MIN * emax2);
-  sturm_bisect_kernel<<<(unsigned)((n + 127) / 128), 128, 0, ctx->stream>>>((int)n, D, E, gl, gu, pivmin, w);
+  ArenaFrame frame(ctx);
+  double *bnd = arena_alloc_n<double>(ctx, 3);
+  if (!bnd) return EVD_E_NOMEM;
+  gershgorin_kernel<<<1, 256, 0, ctx->stream>>>((int)n, D, E, bnd);
+  EVD_LAUNCHED(ctx);
+  sturm_bisect_kernel<<<(unsigned)((n + 127) / 128), 128, 0, ctx->stream>>>((int)n, D, E, bnd, w);
   EVD_LAUNCHED(ctx);
   return 0;
 }
@@ -105,12 +131,19 @@ static dim3 grid2d(long rows, long cols) {
   return dim3((unsigned)std::min<long>((rows + 255) / 256, 64), (unsigned)std::min<long>(cols, 8192));
 }
 
-// symmetric fill of the upper triangle from the lower one + non-finite scan
+// Symmetric fill of the upper triangle from the lower one + non-finite
+// screen.  A non-finite entry raises EVD_E_NONFINITE in the error word (read
+// when the call ends) and is replaced by 0, so the rest of the solve runs on
+// finite data without a host round trip after the screen.
 __global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda, unsigned *__restrict__ err) {
   for (long c = blockIdx.y; c < n; c += gridDim.y)
     for (long r = c + blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
-      const float v = A[r + c * lda];
-      if (!isfinite(v)) atomicOr(err, (unsigned)EVD_E_NONFINITE);
+      float v = A[r + c * lda];
+      if (!isfinite(v)) {
+        atomicOr(err, (unsigned)EVD_E_NONFINITE);
+        v = 0.f;
+        A[r + c * lda] = v;
+      }
       A[c + r * lda] = v;
     }
 }
@@ -140,7 +173,7 @@ size_t syevd_workspace(long n, long b, long nb, char jobz, int precision) {
 }
 
 static void stage_mark(evd_ctx *ctx, int i) {
-  if (ctx->timing) hipEventRecord(ctx->ev[i], ctx->stream);
+  if (ctx->timing && !ctx->capturing) hipEventRecord(ctx->ev[i], ctx->stream);
 }
 
 int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz) {
@@ -161,7 +194,6 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
   // full symmetric storage from the lower triangle, NaN/Inf screen
   symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A, lda, ctx->ctrl + kCtrlError);
   EVD_LAUNCHED(ctx);
-  EVD_TRY(check_device_error(ctx, "input screen"));
 
   stage_mark(ctx, 0);
   EVD_TRY(sy2sb(ctx, n, b, nb, A, lda, W, n, Y, n, wc));
@@ -186,6 +218,7 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
     stage_mark(ctx, 4);
     stage_mark(ctx, 5);
   }
+  if (ctx->capturing) return 0;  // the graph's owner reads the error word after a replay
   EVD_TRY(check_device_error(ctx, "evd_syevd_f32"));
   if (ctx->timing) {
     for (int i = 0; i < EVD_STAGE_COUNT; i++) {
@@ -247,7 +280,6 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
     symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A + q * a_stride, lda, ctx->ctrl + kCtrlError);
     EVD_LAUNCHED(ctx);
   }
-  EVD_TRY(check_device_error(ctx, "input screen"));
 
   stage_mark(ctx, 0);
   for (int q = 0; q < count; q++) {
@@ -361,6 +393,12 @@ extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, f
   EVD_ARG(ctx, !vec || Z != nullptr || n == 0, 9);
   EVD_ARG(ctx, !vec || ldz >= std::max<long>(1, n), 10);
   if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_f32 cannot nest");
+  return syevd_call(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+}
+
+// evd_syevd_f32 after argument checks: tuning knobs, workspace, solve.
+int evd::syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z,
+                    long ldz) {
   // knobs left at 0 (and nb = 0) come from the tuning cache, else defaults
   const Tuning user = ctx->tune;
   int tnb = 0;

@@ -198,6 +198,25 @@ size_t evd_syevd_batched_workspace_f32(int count, long n, long b, long nb, char 
 int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride,
                           float *U, long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride);
 
+/* ---- captured solves (no counterpart in the reference, which launches every
+ * kernel from the host on each call) ----
+ * evd_graph_create runs one evd_syevd_f32 on the given buffers (so A is
+ * consumed and w, Z receive its results), then captures the same solve on
+ * those buffers as a hipGraph.  evd_graph_launch replays it on the context's
+ * stream (asynchronous; write the next matrix into A first, stream-ordered);
+ * evd_graph_wait synchronises the stream and reports the device error word
+ * (e.g. EVD_E_NONFINITE for a non-finite input, whose entries were zeroed).
+ * While a graph lives, the context's arena cannot grow (a larger solve on the
+ * context fails with EVD_E_UNSUPPORTED) and evd_ctx_destroy refuses; other
+ * solves on the context may run between replays.  Results of a replay equal
+ * evd_syevd_f32 on the same input bit for bit. */
+typedef struct evd_graph evd_graph;
+int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z,
+                     long ldz, evd_graph **out);
+int evd_graph_launch(evd_graph *g);
+int evd_graph_wait(evd_graph *g);
+int evd_graph_destroy(evd_graph *g);
+
 /* ---- autotuning (no counterpart in the reference: its tile shapes are
  * compile-time template arguments and the myEVD nb argument) ----
  * Knob vector: {nb, chase_groups, bc_valu, sbr_back_block} (0 = default).
