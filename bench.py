@@ -117,6 +117,68 @@ def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 
             "poisson": {"rate_per_s": rate, "one_at_a_time": poi1, "engine_4slots_batching": poi4}}
 
 
+def captured_bench(E, torch, reps: int = 20):
+    """Small solves launched kernel by kernel from the host (evd_syevd_f32,
+    evd_syevd_batched_f32) against the same solves replayed from a captured
+    hipGraph (evd_graph_*): mean wall time per call over `reps` sequential
+    calls, each restoring its input and waiting for its result."""
+    out = {}
+    ctx = E.Context(0)
+    gctx = E.Context(0)
+    try:
+        for n in (256, 1024):
+            M = E.gen_symmetric(ctx, n, 7, 0)
+            A = E.colmajor(n, n)
+            for _ in range(3):
+                A.copy_(M)
+                E.syevd(ctx, A, b=32, nb=256, overwrite=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                A.copy_(M)
+                E.syevd(ctx, A, b=32, nb=256, overwrite=True)
+            torch.cuda.synchronize()
+            eager = (time.perf_counter() - t0) / reps
+            g = E.SolveGraph(gctx, M, b=32, nb=256)
+            try:
+                for _ in range(3):
+                    g.solve(M)
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    g.solve(M)
+                replay = (time.perf_counter() - t0) / reps
+            finally:
+                g.close()
+            out[f"n{n}_jobzV"] = {"eager_ms": eager * 1e3, "graph_ms": replay * 1e3, "speedup": eager / replay}
+        count, n = 32, 256
+        mats = [E.gen_symmetric(ctx, n, 100 + q, 0) for q in range(count)]
+        for _ in range(2):
+            E.syevd_batched(ctx, mats, b=32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps // 4):
+            E.syevd_batched(ctx, mats, b=32)
+        torch.cuda.synchronize()
+        eager = (time.perf_counter() - t0) / (reps // 4)
+        g = E.BatchedSolveGraph(gctx, mats, b=32)
+        try:
+            for _ in range(2):
+                g.solve(mats)
+            t0 = time.perf_counter()
+            for _ in range(reps // 4):
+                g.solve(mats)
+            replay = (time.perf_counter() - t0) / (reps // 4)
+        finally:
+            g.close()
+        out[f"batch{count}_n{n}_jobzV"] = {"eager_ms": eager * 1e3, "graph_ms": replay * 1e3,
+                                           "speedup": eager / replay,
+                                           "solves_per_s_graph": count / replay}
+    finally:
+        gctx.close()
+        ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +191,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP16/BF16 split) runs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
+    ap.add_argument("--no-captured", action="store_true", help="skip the eager-vs-graph small-solve sample")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--cpu-n", type=int, default=4096)
     args = ap.parse_args()
@@ -241,6 +304,10 @@ def main():
     if not args.no_serve:
         serving = serving_bench(E, torch, args.serve_requests)
         print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
+    captured = None
+    if not args.no_captured:
+        captured = captured_bench(E, torch)
+        print(f"[bench] captured: {captured}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     line = {
         "metric": "EVD s & TFLOP/s at n=16384 FP32 b=32; mixed-size solves/s + p99 latency",
@@ -266,6 +333,7 @@ def main():
         "accuracy": acc,
         "mixed_precision": mixed or None,
         "serving": serving,
+        "captured": captured,
     }
     print(json.dumps(line))
 

@@ -22,7 +22,7 @@ __all__ = [
     "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "hou_tsqr_panel", "tc_ozimmu_syr2k", "band_pack",
     "sb2st", "stedc", "computerWYFromSlide", "computerQFromWY", "my_BC_back_trans_v8_10_noBandU", "bc_back",
     "sbr_back", "checkOrthogonality", "backward_error", "sgemm", "dgemm", "gen_symmetric", "gen_spectrum",
-    "STAGES", "autotune", "syevd_batched", "SolveGraph", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
+    "STAGES", "autotune", "syevd_batched", "SolveGraph", "BatchedSolveGraph", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
 ]
 
 # GEMM precision of SBR and the back transformation (evd_ctx_set_precision)
@@ -422,6 +422,50 @@ class SolveGraph:
             self.close()
         except Exception:
             pass
+
+
+class BatchedSolveGraph(SolveGraph):
+    """`count` same-size solves (evd_syevd_batched_f32) captured as one
+    hipGraph on fixed buffers: A (count, n, n) column-major per problem, w
+    (count, n), Z (count, n, n) with Z[q] the eigenvectors of problem q
+    (columns).  `solve(mats)` copies the matrices in, replays and waits."""
+
+    def __init__(self, ctx: Context, mats, b: int = 32, nb: int = 0, jobz: str = "V"):
+        torch = _torch()
+        self.ctx = ctx
+        self.count = count = len(mats)
+        self.n = n = mats[0].shape[0]
+        self.jobz = jobz
+        dev = mats[0].device
+        self._A = torch.empty((count, n, n), dtype=torch.float32, device=dev)  # row-major of A^T = column-major A
+        self._copy_in(mats)
+        self._w = torch.empty((count, n), dtype=torch.float64, device=dev)
+        self._Z = torch.empty((count, n, n), dtype=torch.float32, device=dev) if jobz == "V" else None
+        h = C.c_void_p()
+        ctx.call("evd_graph_create_batched", _ch(jobz), count, n, b, nb, C.c_void_p(self._A.data_ptr()), n, n * n,
+                 C.c_void_p(self._w.data_ptr()), n, C.c_void_p(self._Z.data_ptr() if self._Z is not None else 0), n,
+                 n * n, C.byref(h))
+        self.h = h
+
+    def _copy_in(self, mats):
+        for q, M in enumerate(mats):
+            self._A[q].copy_(M.t())
+
+    @property
+    def w(self):
+        return self._w
+
+    @property
+    def Z(self):
+        return self._Z.transpose(1, 2) if self._Z is not None else None
+
+    def solve(self, mats):
+        if len(mats) != self.count:
+            raise ValueError(f"graph was captured for {self.count} problems")
+        self._copy_in(mats)
+        self.launch()
+        self.wait()
+        return self.w, self.Z
 
 
 def syevd_workspace(n: int, b: int, nb: int, jobz: str = "V") -> int:

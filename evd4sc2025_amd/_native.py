@@ -52,6 +52,7 @@ SIGNATURES = {
     "evd_syevd_batched_workspace_f32": (_sz, [_i, _l, _l, _l, _ch, _i]),
     "evd_sb2st_batched_f64": (_i, [_vp, _i, _l, _l, _vp, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l]),
     "evd_graph_create": (_i, [_vp, _ch, _l, _l, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_vp)]),
+    "evd_graph_create_batched": (_i, [_vp, _ch, _i, _l, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l, _l, C.POINTER(_vp)]),
     "evd_graph_launch": (_i, [_vp]),
     "evd_graph_wait": (_i, [_vp]),
     "evd_graph_destroy": (_i, [_vp]),

@@ -65,6 +65,7 @@ struct evd_ctx {
   std::map<long, evd::DcPlan> dc_plans;  // tridiagonal merge plans by order
   bool capturing = false;                 // inside hipGraph capture: no host round trips
   int graphs = 0;                         // live captured graphs: the arena may not move
+  long shape_div = 1;                     // GEMM launches carry this many problems' products (batched SBR)
 
   bool timing = false;
   hipEvent_t ev[EVD_STAGE_COUNT + 1] = {};
@@ -147,6 +148,8 @@ int dgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, double alpha, 
           const double *B, long ldb, double beta, double *C, long ldc);
 int syr2k_lower_mirror(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
                        float beta, float *C, long ldc);
+int syr2k_lower_mirror_batched(evd_ctx *ctx, int count, long n, long k, float alpha, const float *A, long lda,
+                               long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC);
 
 // grouped FP64 GEMM: C[:, colmap[j]] = A (m x k) * B (k x n) per problem
 struct DGemmProblem {
@@ -170,6 +173,8 @@ int check_device_error(evd_ctx *ctx, const char *where);
 
 // evd_syevd_f32 after its argument checks (syevd.hip)
 int syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz);
+int syevd_batched_call(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda, long a_stride,
+                       double *w, long w_stride, float *Z, long ldz, long z_stride);
 
 }  // namespace evd
 

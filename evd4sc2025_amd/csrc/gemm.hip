@@ -27,6 +27,14 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBK = 16;
 
+// Batch count that tile-shape and split-K choices see: a launch that carries
+// the same product for ctx->shape_div independent problems is shaped like the
+// launch of one problem's products, so each problem's arithmetic (the K
+// split, hence the summation order) is that of its single solve.
+static inline long shape_batch(const evd_ctx *ctx, long batch) {
+  return (ctx->shape_div > 1 && batch % ctx->shape_div == 0) ? batch / ctx->shape_div : batch;
+}
+
 template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, int BK = kBK>
 struct TileLoader {
   static constexpr int NA = BM * BK / 256;
@@ -222,7 +230,7 @@ template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK>
 static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
                         const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
                         long min_chunk = 256) {
-  long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
+  long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * shape_batch(ctx, batch);
   int ksplit = 1;
   const long target = 2L * ctx->num_cus;
   // split K while the tiles alone leave the chip idle, keeping each chunk at
@@ -263,7 +271,8 @@ static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, con
   if (n <= 32 && m > 64) {
     // tall-skinny: the row tile is chosen for parallelism first (the K
     // dimension of these products is often short), then K is split
-    const long t256 = (m + 255) / 256 * batch, t128 = (m + 127) / 128 * batch;
+    const long hb = shape_batch(ctx, batch);
+    const long t256 = (m + 255) / 256 * hb, t128 = (m + 127) / 128 * hb;
     if (t256 >= target / 2 || k >= 4096)
       return launch_sgemm<TA, TB, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     if (t128 >= target / 2)
@@ -273,7 +282,7 @@ static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, con
   }
   if (m <= 32 && n > 64)
     return launch_sgemm<TA, TB, 32, 256, 1, 4>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
-  long big_tiles = ((m + 127) / 128) * ((n + 127) / 128) * batch;
+  long big_tiles = ((m + 127) / 128) * ((n + 127) / 128) * shape_batch(ctx, batch);
   if (big_tiles >= target || (m > 64 && n > 64 && k < 1024))
     return launch_sgemm<TA, TB, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   return launch_sgemm<TA, TB, 64, 64, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128);
@@ -318,10 +327,14 @@ int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, c
 
 // ----------------------------------------------------------------- SYR2K
 // Lower tiles (ti >= tj) of C; the K loop runs over [A|B] against [B|A].
+// blockIdx.y = problem of a same-size batch (operands at the strides).
 __global__ __launch_bounds__(256) void syr2k_kernel(int n, int k, float alpha, const float *__restrict__ A, long lda,
                                                     const float *__restrict__ B, long ldb, float beta,
-                                                    float *__restrict__ C, long ldc) {
+                                                    float *__restrict__ C, long ldc, long sA, long sB, long sC) {
   constexpr int BM = 128, BN = 128, WM = 2, WN = 2, MI = 4, NI = 4, BK = kBK;
+  A += blockIdx.y * sA;
+  B += blockIdx.y * sB;
+  C += blockIdx.y * sC;
   using L = TileLoader<false, true, BM, BN, WM, WN, float, BK>;
   __shared__ float As[2 * BK * L::LDSA];
   __shared__ float Bs[2 * BK * L::LDSB];
@@ -365,11 +378,21 @@ static int syr2k_split(evd_ctx *ctx, long n, long k, float alpha, const float *A
 
 int syr2k_lower_mirror(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,
                        float beta, float *C, long ldc) {
-  if (n <= 0) return 0;
-  if (ctx->precision != EVD_PREC_FP32) return syr2k_split(ctx, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  return syr2k_lower_mirror_batched(ctx, 1, n, k, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0);
+}
+
+int syr2k_lower_mirror_batched(evd_ctx *ctx, int count, long n, long k, float alpha, const float *A, long lda,
+                               long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC) {
+  if (n <= 0 || count <= 0) return 0;
+  if (ctx->precision != EVD_PREC_FP32) {
+    for (int q = 0; q < count; q++)
+      EVD_TRY(syr2k_split(ctx, n, k, alpha, A + q * sA, lda, B + q * sB, ldb, beta, C + q * sC, ldc));
+    return 0;
+  }
   long nt = (n + 127) / 128;
   long tiles = nt * (nt + 1) / 2;
-  syr2k_kernel<<<(unsigned)tiles, 256, 0, ctx->stream>>>((int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+  syr2k_kernel<<<dim3((unsigned)tiles, (unsigned)count), 256, 0, ctx->stream>>>((int)n, (int)k, alpha, A, lda, B, ldb,
+                                                                                 beta, C, ldc, sA, sB, sC);
   EVD_LAUNCHED(ctx);
   return 0;
 }
@@ -605,7 +628,7 @@ __global__ __launch_bounds__(256) void syr2k_split_kernel(int n, int k, float al
 template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN>
 static int launch_sgemm_split(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
                               const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch) {
-  const long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
+  const long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * shape_batch(ctx, batch);
   int ksplit = 1;
   const long target = 2L * ctx->num_cus;
   if (tiles < target && k >= 512) {

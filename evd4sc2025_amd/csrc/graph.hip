@@ -16,18 +16,14 @@ struct evd_graph {
 
 using namespace evd;
 
-extern "C" int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w,
-                                float *Z, long ldz, evd_graph **out) {
-  if (!ctx) return -1;
-  EVD_ARG(ctx, out != nullptr, 11);
-  EVD_ARG(ctx, n >= 1, 3);
-  *out = nullptr;
-  // eager solve: checks the arguments, sizes the arena, builds the merge plan
-  EVD_TRY(evd_syevd_f32(ctx, jobz, n, b, nb, A, lda, w, Z, ldz));
+namespace {
 
-  // Capture on a private non-blocking stream (the context's stream may be the
-  // null stream, which cannot be captured); replays go to the context's
-  // stream.
+// Capture `solve` (which enqueues the work on ctx->stream) as a graph.  The
+// capture runs on a private non-blocking stream (the context's stream may be
+// the null stream, which cannot be captured); replays go to the context's
+// stream.
+template <typename F>
+int capture(evd_ctx *ctx, F solve, evd_graph **out) {
   hipStream_t cap = nullptr;
   EVD_HIP(ctx, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   hipStream_t saved_stream = ctx->stream;
@@ -41,7 +37,7 @@ extern "C" int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb
   if (e != hipSuccess) {
     status = set_error(ctx, EVD_E_HIP, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
   } else {
-    status = syevd_call(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+    status = solve();
     e = hipStreamEndCapture(cap, &graph);
     if (status == 0 && e != hipSuccess)
       status = set_error(ctx, EVD_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
@@ -67,6 +63,36 @@ extern "C" int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb
   ctx->graphs++;
   *out = g;
   return 0;
+}
+
+}  // namespace
+
+extern "C" int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w,
+                                float *Z, long ldz, evd_graph **out) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, out != nullptr, 11);
+  EVD_ARG(ctx, n >= 1, 3);
+  *out = nullptr;
+  // eager solve: checks the arguments, sizes the arena, builds the merge plan
+  EVD_TRY(evd_syevd_f32(ctx, jobz, n, b, nb, A, lda, w, Z, ldz));
+  return capture(ctx, [&] { return syevd_call(ctx, jobz, n, b, nb, A, lda, w, Z, ldz); }, out);
+}
+
+extern "C" int evd_graph_create_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A,
+                                        long lda, long a_stride, double *w, long w_stride, float *Z, long ldz,
+                                        long z_stride, evd_graph **out) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, out != nullptr, 15);
+  EVD_ARG(ctx, count >= 1, 3);
+  EVD_ARG(ctx, n >= 1, 4);
+  *out = nullptr;
+  EVD_TRY(evd_syevd_batched_f32(ctx, jobz, count, n, b, nb, A, lda, a_stride, w, w_stride, Z, ldz, z_stride));
+  return capture(
+      ctx,
+      [&] {
+        return syevd_batched_call(ctx, jobz, count, n, b, nb, A, lda, a_stride, w, w_stride, Z, ldz, z_stride);
+      },
+      out);
 }
 
 extern "C" int evd_graph_launch(evd_graph *g) {

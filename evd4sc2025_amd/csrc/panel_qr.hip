@@ -69,12 +69,19 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
                                                        long ldw, float *__restrict__ R, long ldr,
                                                        float *__restrict__ Aout, long ldo, double *__restrict__ part,
                                                        double *__restrict__ tau_out, unsigned *__restrict__ tags,
-                                                       unsigned *__restrict__ epoch_word, unsigned *__restrict__ err) {
+                                                       unsigned *__restrict__ epoch_word, unsigned *__restrict__ err,
+                                                       long s_in, long s_y, long s_w, long s_out) {
   constexpr int CH = NB < 32 ? NB : 32;  // slots per in-wave reduction
   constexpr int PASSG = 32;              // group records gathered per pass
   static_assert(NB <= 256, "the pivot rows must live in group 0");
   const int G = gridDim.x;
   const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // same-size panels of several problems (one workgroup each, G = 1):
+  // problem blockIdx.y at the given strides
+  Ain += blockIdx.y * s_in;
+  Y += blockIdx.y * s_y;
+  W += blockIdx.y * s_w;
+  if (Aout) Aout += blockIdx.y * s_out;
   const int k = min(m, nc);
   const long rec = 2 * NB;  // exchange record: NB partial sums, NB pivot-row entries
   __shared__ double red[4][NB];
@@ -351,9 +358,12 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
 
 template <int NB, int RPT>
 static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, long ldy, float *W,
-                        long ldw, float *R, long ldr, float *Aout, long ldo, double *tau_out) {
+                        long ldw, float *R, long ldr, float *Aout, long ldo, double *tau_out, int count = 1,
+                        long s_in = 0, long s_y = 0, long s_w = 0, long s_out = 0) {
   const long rows_per = 256L * RPT;
   int G = (int)((m + rows_per - 1) / rows_per);
+  if (count > 1 && (G != 1 || R || tau_out))
+    return set_error(ctx, EVD_E_UNSUPPORTED, "batched panels need one workgroup per problem and no R / tau output");
   int maxres = 0;
   EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<NB, RPT>, 256, 0, &maxres));
   if (G > maxres) return set_error(ctx, EVD_E_UNSUPPORTED, "panel of %ld rows needs %d co-resident groups (max %d)", m, G, maxres);
@@ -365,12 +375,13 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
   unsigned *epoch_word = ctx->ctrl + kCtrlEpoch;
   unsigned *err = ctx->ctrl + kCtrlError;
   int mi = (int)m, ni = (int)nc;
-  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R,
-                  &ldr, &Aout, &ldo, &part, &tau_out, &tags, &epoch_word, &err};
+  void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &tau_out,
+                  &tags, &epoch_word, &err, &s_in, &s_y, &s_w, &s_out};
   // one workgroup is trivially co-resident: a regular launch, which (unlike a
   // cooperative one) may overlap with the work of other streams
   if (G == 1)
-    EVD_HIP(ctx, hipLaunchKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0, ctx->stream));
+    EVD_HIP(ctx, hipLaunchKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G, count), dim3(256), args, 0,
+                                 ctx->stream));
   else
     EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0,
                                             ctx->stream));
@@ -384,6 +395,28 @@ int panel_qr(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, 
   if (nc <= 32) return launch_panel<32, 2>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
   if (nc <= 64) return launch_panel<64, 1>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
   return set_error(ctx, EVD_E_UNSUPPORTED, "panel width %ld > 64", nc);
+}
+
+// Same-size panels of `count` problems (A, Y, W of problem q at q times the
+// strides; the factored block is written back into A).  Panels of one
+// workgroup (m <= 512 rows for 32 columns, 256 for 64) share one launch; taller
+// panels run per problem.  Each problem's arithmetic is that of panel_qr.
+int panel_qr_batched(evd_ctx *ctx, int count, long m, long nc, float *A, long lda, long sA, float *Y, long ldy,
+                     long sY, float *W, long ldw, long sW) {
+  if (m <= 0 || nc <= 0 || count <= 0) return 0;
+  const long rows_per = nc <= 32 ? 512 : 256;
+  if (count > 1 && m <= rows_per) {
+    if (nc <= 32)
+      return launch_panel<32, 2>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA, sY, sW,
+                                 sA);
+    if (nc <= 64)
+      return launch_panel<64, 1>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA, sY, sW,
+                                 sA);
+  }
+  for (int q = 0; q < count; q++)
+    EVD_TRY(panel_qr(ctx, m, nc, A + q * sA, lda, Y + q * sY, ldy, W + q * sW, ldw, nullptr, 0, A + q * sA, lda,
+                     nullptr));
+  return 0;
 }
 
 }  // namespace evd

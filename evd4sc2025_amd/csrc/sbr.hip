@@ -25,6 +25,8 @@ namespace evd {
 
 int panel_qr(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, long ldy, float *W, long ldw,
              float *R, long ldr, float *Aout, long ldo, double *tau_out);
+int panel_qr_batched(evd_ctx *ctx, int count, long m, long nc, float *A, long lda, long sA, float *Y, long ldy,
+                     long sY, float *W, long ldw, long sW);
 
 size_t sy2sb_workspace(long n, long b, long nb) {
   size_t s = 0;
@@ -89,6 +91,80 @@ int sy2sb(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, l
   return 0;
 }
 
+// Same-size SBR of `count` problems (A, W, Y of problem q at q times the
+// strides): the panel loop of sy2sb with every kernel launched once for all
+// problems.  GEMM shapes and K splits are chosen as for one problem
+// (ctx->shape_div), so each problem's result equals its sy2sb bit for bit.
+size_t sy2sb_batched_workspace(int count, long n, long b, long nb) {
+  size_t per = align256((size_t)n * nb * sizeof(float)) + align256((size_t)nb * b * sizeof(float)) * 2;
+  // split-K partials: the K split of a skinny product stays below n / 128
+  // chunks, so one problem's partials stay below 2 n^2 + (n / 64 + 1) nb b words
+  size_t part = (size_t)2 * n * n * sizeof(float) + (size_t)(n / 64 + 1) * nb * b * sizeof(float);
+  return (per + part) * (size_t)count + ((size_t)8 << 20);
+}
+
+int sy2sb_batched(evd_ctx *ctx, int count, long n, long b, long nb, float *A, long lda, long sA, float *W, long ldw,
+                  long sW, float *Y, long ldy, long sY, long wcols) {
+  if (n <= 0 || count <= 0) return 0;
+  if (count == 1) return sy2sb(ctx, n, b, nb, A, lda, W, ldw, Y, ldy, wcols);
+  for (int q = 0; q < count; q++) {
+    EVD_TRY(fill_f32(ctx, W + q * sW, n, wcols, ldw, 0.f));
+    EVD_TRY(fill_f32(ctx, Y + q * sY, n, wcols, ldy, 0.f));
+  }
+  ArenaFrame frame(ctx);
+  struct ShapeDiv {
+    evd_ctx *c;
+    long saved;
+    ShapeDiv(evd_ctx *c_, long d) : c(c_), saved(c_->shape_div) { c->shape_div = d; }
+    ~ShapeDiv() { c->shape_div = saved; }
+  } shape(ctx, count);
+  const long ldz = n, sZ = n * nb, sT = nb * b, sT3 = b * b;
+  float *Z = arena_alloc_n<float>(ctx, (size_t)sZ * count);
+  float *t = arena_alloc_n<float>(ctx, (size_t)sT * count);
+  float *t3 = arena_alloc_n<float>(ctx, (size_t)sT3 * count);
+  if (!Z || !t || !t3) return EVD_E_NOMEM;
+  const long P = nb / b;
+  long c0 = 0;
+  while (n - c0 - b > 1) {
+    long p = 0;
+    for (;; p++) {
+      const long c = c0 + p * b, r0 = c + b, m = n - r0;
+      float *Ap = A + r0 + c * lda;
+      float *Wp = W + r0 + c * ldw;
+      float *Yp = Y + r0 + c * ldy;
+      float *Zp = Z + r0 + p * b * ldz;
+      EVD_TRY(panel_qr_batched(ctx, count, m, b, Ap, lda, sA, Yp, ldy, sY, Wp, ldw, sW));
+      EVD_TRY(sgemm(ctx, 'N', 'N', m, b, m, 1.f, A + r0 + r0 * lda, lda, Wp, ldw, 0.f, Zp, ldz, count, sA, sW, sZ));
+      if (p > 0) {
+        const long pb = p * b;
+        const float *Yq = Y + r0 + c0 * ldy;
+        const float *Zq = Z + r0;
+        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, t, pb, count, sZ, sW, sT));
+        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, pb, -1.f, Yq, ldy, t, pb, 1.f, Zp, ldz, count, sY, sT, sZ));
+        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, t, pb, count, sY, sW, sT));
+        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, pb, -1.f, Zq, ldz, t, pb, 1.f, Zp, ldz, count, sZ, sT, sZ));
+      }
+      EVD_TRY(sgemm(ctx, 'T', 'N', b, b, m, 1.f, Wp, ldw, Zp, ldz, 0.f, t3, b, count, sW, sZ, sT3));
+      EVD_TRY(sgemm(ctx, 'N', 'N', m, b, b, -0.5f, Yp, ldy, t3, b, 1.f, Zp, ldz, count, sY, sT3, sZ));
+      const long kk = (p + 1) * b;
+      const float *Ya = Y + r0 + c0 * ldy;
+      const float *Za = Z + r0;
+      const bool more = (p + 1 < P) && (n - (r0 + b) > 1);
+      if (more) {
+        float *An = A + r0 + r0 * lda;
+        EVD_TRY(sgemm(ctx, 'N', 'T', m, b, kk, -1.f, Ya, ldy, Za, ldz, 1.f, An, lda, count, sY, sZ, sA));
+        EVD_TRY(sgemm(ctx, 'N', 'T', m, b, kk, -1.f, Za, ldz, Ya, ldy, 1.f, An, lda, count, sZ, sY, sA));
+        continue;
+      }
+      EVD_TRY(syr2k_lower_mirror_batched(ctx, count, m, kk, -1.f, Ya, ldy, sY, Za, ldz, sZ, 1.f, A + r0 + r0 * lda,
+                                         lda, sA));
+      break;
+    }
+    c0 += (p + 1) * b;
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------- WY merge
 // Doubling merge of the panel WY pairs inside each blk-wide block (batched
 // over the blocks' pairs).  W and Y must have `wcols` (a multiple of blk)
@@ -142,6 +218,59 @@ int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float
   if (wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
   EVD_TRY(wy_merge_doubling(ctx, n, b, blk, wcols, W, ldw, Y, ldy));
   return sbr_apply(ctx, n, b, blk, wcols, ncols, Z, ldz, W, ldw, Y, ldy);
+}
+
+// Same-size back transformation of `count` problems: W, Y of problem q at
+// q (ldw wcols) / q (ldy wcols) (contiguous, so the pairwise merges of all
+// problems form one strided batch), Z at q sZ.  Shapes as for one problem.
+int sbr_back_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float *Z, long ldz, long sZ, float *W,
+                     long ldw, const float *Y, long ldy, long wcols) {
+  if (count == 1) return sbr_back(ctx, n, b, ncols, Z, ldz, W, ldw, Y, ldy, wcols);
+  long blk = ctx->tune.sbr_back_block > 0 ? ctx->tune.sbr_back_block : 1024;
+  while (blk > b && blk > wcols) blk /= 2;
+  if (blk < b) blk = b;
+  long bb = b;
+  while (bb * 2 <= blk) bb *= 2;
+  blk = bb;
+  if (wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
+  ArenaFrame frame(ctx);
+  struct ShapeDiv {
+    evd_ctx *c;
+    long saved;
+    ShapeDiv(evd_ctx *c_, long d) : c(c_), saved(c_->shape_div) { c->shape_div = d; }
+    ~ShapeDiv() { c->shape_div = saved; }
+  } shape(ctx, count);
+  const long sW = ldw * wcols, sY = ldy * wcols;
+  // merges: the pairs of every problem are one batch (pair p of problem q is
+  // pair q * npairs + p at stride 2 cw ld)
+  float *T = arena_alloc_n<float>(ctx, (size_t)(wcols / 2) * blk * count);
+  if (!T) return EVD_E_NOMEM;
+  for (long cw = b; cw < blk; cw *= 2) {
+    const long npairs = wcols / (2 * cw);
+    if (npairs <= 0) break;
+    const long nb_all = npairs * count;
+    EVD_TRY(sgemm(ctx, 'T', 'N', cw, cw, n, 1.f, Y, ldy, W + cw * ldw, ldw, 0.f, T, cw, nb_all, 2 * cw * ldy,
+                  2 * cw * ldw, cw * cw));
+    EVD_TRY(sgemm(ctx, 'N', 'N', n, cw, cw, -1.f, W, ldw, T, cw, 1.f, W + cw * ldw, ldw, nb_all, 2 * cw * ldw, cw * cw,
+                  2 * cw * ldw));
+  }
+  // Z <- Q_SBR Z block by block from the last block
+  const long sT1 = blk * ncols;
+  float *T1 = arena_alloc_n<float>(ctx, (size_t)sT1 * count);
+  if (!T1) return EVD_E_NOMEM;
+  const long nblk = (wcols + blk - 1) / blk;
+  for (long k = nblk - 1; k >= 0; k--) {
+    const long B0 = k * blk;
+    const long bw = std::min(blk, wcols - B0);
+    const long r0 = B0 + b;
+    if (r0 >= n) continue;
+    const long m = n - r0;
+    EVD_TRY(sgemm(ctx, 'T', 'N', bw, ncols, m, 1.f, Y + r0 + B0 * ldy, ldy, Z + r0, ldz, 0.f, T1, bw, count, sY, sZ,
+                  sT1));
+    EVD_TRY(sgemm(ctx, 'N', 'N', m, ncols, bw, -1.f, W + r0 + B0 * ldw, ldw, T1, bw, 1.f, Z + r0, ldz, count, sW, sT1,
+                  sZ));
+  }
+  return 0;
 }
 
 }  // namespace evd

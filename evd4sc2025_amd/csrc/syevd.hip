@@ -35,6 +35,11 @@ int bc_back_wy_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, lo
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols);
+size_t sy2sb_batched_workspace(int count, long n, long b, long nb);
+int sy2sb_batched(evd_ctx *ctx, int count, long n, long b, long nb, float *A, long lda, long sA, float *W, long ldw,
+                  long sW, float *Y, long ldy, long sY, long wcols);
+int sbr_back_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float *Z, long ldz, long sZ, float *W,
+                     long ldw, const float *Y, long ldy, long wcols);
 
 static long wy_cols(long n) { return (n + 1023) / 1024 * 1024; }
 
@@ -245,10 +250,14 @@ size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, in
   size_t per = align256((size_t)n * wc * sizeof(float)) * 2 + align256((size_t)2 * b * n * sizeof(double)) +
                align256((size_t)(n + b) * n * sizeof(float)) + align256((size_t)n * sizeof(double)) * 2;
   size_t s = per * count + 4 * 256;
-  size_t later = std::max(sy2sb_workspace(n, b, nb), sb2st_batched_workspace(count, n) + ((size_t)1 << 20));
+  size_t later = std::max(sy2sb_batched_workspace(count, n, b, nb), sb2st_batched_workspace(count, n) + ((size_t)1 << 20));
   if (vec) {
     s += align256((size_t)n * n * sizeof(double));
     size_t back = syevd_workspace(n, b, nb, jobz, precision);  // per-problem back transformation and D&C bound
+    // batched SBR back transformation: merge products, block products, partials
+    back = std::max(back, sy2sb_batched_workspace(count, n, b, nb) +
+                              (align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
+                               align256((size_t)1024 * n * sizeof(float))) * (size_t)count);
     if (b == 32) back = std::max(back, bc_back_wy_bytes(n) * count + ((size_t)1 << 20));
     later = std::max(later, back);
   }
@@ -261,12 +270,11 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   if (n == 0 || count == 0) return 0;
   ArenaFrame frame(ctx);
   const long wc = wy_cols(n);
-  std::vector<float *> W(count), Y(count);
-  for (int q = 0; q < count; q++) {
-    W[q] = arena_alloc_n<float>(ctx, (size_t)n * wc);
-    Y[q] = arena_alloc_n<float>(ctx, (size_t)n * wc);
-    if (!W[q] || !Y[q]) return EVD_E_NOMEM;
-  }
+  // W, Y of all problems contiguous (problem q at q n wc)
+  const long sWY = n * wc;
+  float *W = arena_alloc_n<float>(ctx, (size_t)sWY * count);
+  float *Y = arena_alloc_n<float>(ctx, (size_t)sWY * count);
+  if (!W || !Y) return EVD_E_NOMEM;
   const long ldband = 2 * b, band_stride = ldband * n, ldu = n + b, u_stride = ldu * n;
   const long e_stride = std::max<long>(n, 1);
   double *band = arena_alloc_n<double>(ctx, (size_t)band_stride * count);
@@ -282,11 +290,8 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   }
 
   stage_mark(ctx, 0);
-  for (int q = 0; q < count; q++) {
-    float *Aq = A + q * a_stride;
-    EVD_TRY(sy2sb(ctx, n, b, nb, Aq, lda, W[q], n, Y[q], n, wc));
-    EVD_TRY(band_pack(ctx, n, b, Aq, lda, band + q * band_stride, ldband));
-  }
+  EVD_TRY(sy2sb_batched(ctx, count, n, b, nb, A, lda, a_stride, W, n, sWY, Y, n, sWY, wc));
+  for (int q = 0; q < count; q++) EVD_TRY(band_pack(ctx, n, b, A + q * a_stride, lda, band + q * band_stride, ldband));
   stage_mark(ctx, 1);
   EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, n, E, e_stride));
   stage_mark(ctx, 2);
@@ -308,7 +313,7 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
         EVD_TRY(bc_back(ctx, n, b, n, Z + q * z_stride, ldz, U + q * u_stride, ldu, true));
     }
     stage_mark(ctx, 4);
-    for (int q = 0; q < count; q++) EVD_TRY(sbr_back(ctx, n, b, n, Z + q * z_stride, ldz, W[q], n, Y[q], n, wc));
+    EVD_TRY(sbr_back_batched(ctx, count, n, b, n, Z, ldz, z_stride, W, n, Y, n, wc));
     stage_mark(ctx, 5);
   } else {
     for (int q = 0; q < count; q++) EVD_TRY(sterf_bisect(ctx, n, D + q * n, E + q * e_stride, w + q * w_stride));
@@ -316,6 +321,7 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
     stage_mark(ctx, 4);
     stage_mark(ctx, 5);
   }
+  if (ctx->capturing) return 0;  // the graph's owner reads the error word after a replay
   EVD_TRY(check_device_error(ctx, "evd_syevd_batched_f32"));
   if (ctx->timing) {
     for (int i = 0; i < EVD_STAGE_COUNT; i++) {
@@ -438,6 +444,12 @@ extern "C" int evd_syevd_batched_f32(evd_ctx *ctx, char jobz, int count, long n,
   EVD_ARG(ctx, !vec || ldz >= std::max<long>(1, n), 13);
   EVD_ARG(ctx, !vec || count <= 1 || z_stride >= ldz * n, 14);
   if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_batched_f32 cannot nest");
+  return syevd_batched_call(ctx, jobz, count, n, b, nb, A, lda, a_stride, w, w_stride, Z, ldz, z_stride);
+}
+
+// evd_syevd_batched_f32 after argument checks: tuning knobs, workspace, solve.
+int evd::syevd_batched_call(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda,
+                            long a_stride, double *w, long w_stride, float *Z, long ldz, long z_stride) {
   const Tuning user = ctx->tune;
   int tnb = 0;
   Tuning t;

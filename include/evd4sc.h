@@ -213,6 +213,11 @@ int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, double *band,
 typedef struct evd_graph evd_graph;
 int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z,
                      long ldz, evd_graph **out);
+/* The same for a same-size batch (evd_syevd_batched_f32 on fixed strided
+ * buffers): one graph per size bucket of a batched workload. */
+int evd_graph_create_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda,
+                             long a_stride, double *w, long w_stride, float *Z, long ldz, long z_stride,
+                             evd_graph **out);
 int evd_graph_launch(evd_graph *g);
 int evd_graph_wait(evd_graph *g);
 int evd_graph_destroy(evd_graph *g);

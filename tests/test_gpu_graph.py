@@ -94,3 +94,30 @@ def test_graph_pins_the_arena(own_ctx):
         g.close()
     r = E.syevd(own_ctx, to_dev(O.gen_symmetric(1024, 2, 0)), b=32, nb=256)
     assert np.isfinite(to_host(r.w)).all()
+
+
+@pytest.mark.parametrize("count,n,jobz", [(8, 200, "V"), (5, 640, "V"), (16, 96, "N")])
+def test_batched_graph_replays_equal_eager(own_ctx, count, n, jobz):
+    import evd4sc2025_amd as E
+
+    torch = pytest.importorskip("torch")
+    print(f"batched graph case count={count} n={n} jobz={jobz}", flush=True)
+    sets = [[to_dev(O.gen_symmetric(n, 1000 * s + q, 0)) for q in range(count)] for s in range(3)]
+    g = E.BatchedSolveGraph(own_ctx, sets[0], b=32, nb=0, jobz=jobz)
+    try:
+        ref_ctx = E.Context(0)
+        try:
+            eager = []
+            for mats in sets:
+                w, Z = E.syevd_batched(ref_ctx, mats, b=32, nb=0, jobz=jobz)
+                torch.cuda.synchronize()
+                eager.append((to_host(w), to_host(Z.contiguous()) if Z is not None else None))
+        finally:
+            ref_ctx.close()
+        for rep, s in enumerate([1, 2, 0, 1]):
+            w, Z = g.solve(sets[s])
+            assert np.array_equal(to_host(w), eager[s][0]), f"replay {rep} (set {s}): eigenvalues differ"
+            if jobz == "V":
+                assert np.array_equal(to_host(Z.contiguous()), eager[s][1]), f"replay {rep} (set {s}): vectors differ"
+    finally:
+        g.close()
