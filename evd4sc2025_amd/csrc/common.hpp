@@ -62,7 +62,7 @@ struct evd_ctx {
   bool tune_cache = true;  // fill knobs left at 0 from the process-wide tuning cache
   int precision = EVD_PREC_FP32;
 
-  std::map<long, evd::DcPlan> dc_plans;  // tridiagonal merge plans by order
+  std::map<long, evd::DcPlan> dc_plans;  // tridiagonal merge plans by order | (batch count << 32)
   bool capturing = false;                 // inside hipGraph capture: no host round trips
   int graphs = 0;                         // live captured graphs: the arena may not move
   long shape_div = 1;                     // GEMM launches carry this many problems' products (batched SBR)

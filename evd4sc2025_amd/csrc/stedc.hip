@@ -53,6 +53,12 @@ struct DCScratch {
   double *tau, *lam, *zhat;
   int *colmap, *dmap;
   double *Qc, *Uc;  // n x n, ld n
+  // Same-size batches run as one concatenated index space: merge starts (and
+  // every vector above) are global, problem q = MergeDesc::pad owns indices
+  // [q n, q n + n).  Its matrices sit at their own offsets; adding
+  // q * zmul (Z) or q * qmul (Qc, Uc) to an index formed from global starts
+  // lands in problem q's matrix (0 for a single problem).
+  long zmul = 0, qmul = 0;
 };
 
 // Tearing of every split at once, one thread per diagonal entry (an entry
@@ -86,7 +92,7 @@ __global__ void dc_sort_kernel(const MergeDesc *__restrict__ md, const double *_
     double d = D[st + t], z;
     int pos;
     if (t < m.n1) {
-      z = Z[(st + m.n1 - 1) + (long)(st + t) * ldz];
+      z = Z[(long)m.pad * sc.zmul + (st + m.n1 - 1) + (long)(st + t) * ldz];
       int lo = 0, hi = m.n2;
       while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -97,7 +103,7 @@ __global__ void dc_sort_kernel(const MergeDesc *__restrict__ md, const double *_
       }
       pos = t + lo;
     } else {
-      z = m.sgn * Z[(st + m.n1) + (long)(st + t) * ldz];
+      z = m.sgn * Z[(long)m.pad * sc.zmul + (st + m.n1) + (long)(st + t) * ldz];
       int lo = 0, hi = m.n1;
       while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -372,7 +378,7 @@ __global__ void dc_rotate_kernel(const MergeDesc *__restrict__ md, const MergeOu
   const MergeOut o = mo[blockIdx.y];
   if (o.nrot == 0) return;
   const int N = m.n1 + m.n2, st = m.start;
-  double *Zb = Z + st + (long)st * ldz;
+  double *Zb = Z + ((long)m.pad * sc.zmul + st + (long)st * ldz);
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
     for (int t = 0; t < o.nrot; t++) {
       const int p = sc.rot_p[st + t], q = sc.rot_n[st + t];
@@ -394,8 +400,8 @@ __global__ void dc_gather_kernel(const MergeDesc *__restrict__ md, const MergeOu
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int c0 = blockIdx.y * 64, c1 = min(N, c0 + 64);
   if (r >= N || c0 >= N) return;
-  const double *Zb = Z + st + (long)st * ldz;
-  double *Qc = sc.Qc + st + (long)st * ldq;
+  const double *Zb = Z + ((long)m.pad * sc.zmul + st + (long)st * ldz);
+  double *Qc = sc.Qc + ((long)m.pad * sc.qmul + st + (long)st * ldq);
   for (int k = c0; k < c1; k++) {
     if (k < o.K)
       Qc[r + (long)sc.qpos[st + k] * ldq] = Zb[r + (long)sc.nd_col[st + k] * ldz];
@@ -580,7 +586,7 @@ __global__ __launch_bounds__(256) void dc_vectors_kernel(const MergeDesc *__rest
   const double *d = sc.nd_d + st;
   const double dorg = d[sc.orig[st + i]];
   const double ti = sc.tau[st + i];
-  double *Uc = sc.Uc + st + (long)st * ldu;
+  double *Uc = sc.Uc + ((long)m.pad * sc.qmul + st + (long)st * ldu);
   __shared__ double red[4];
   double ss = 0.0;
   for (int j = threadIdx.x; j < K; j += 256) {
@@ -642,11 +648,12 @@ __global__ void dc_copy_deflated_kernel(const MergeDesc *__restrict__ md, const 
   const MergeOut o = mo[blockIdx.y];
   const int st = m.start, N = m.n1 + m.n2;
   const long total = (long)N * o.ndefl;
-  const double *Qc = sc.Qc + st + (long)st * ldq;
+  const double *Qc = sc.Qc + ((long)m.pad * sc.qmul + st + (long)st * ldq);
+  const long zsh = (long)m.pad * sc.zmul;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const int r = (int)(e % N);
     const int u = (int)(e / N);
-    Z[(st + r) + (long)sc.dmap[st + u] * ldz] = Qc[r + (long)(o.K + u) * ldq];
+    Z[zsh + (st + r) + (long)sc.dmap[st + u] * ldz] = Qc[r + (long)(o.K + u) * ldq];
   }
 }
 
@@ -663,16 +670,17 @@ __global__ void dc_problems_kernel(int nm, const MergeDesc *__restrict__ md, con
   const MergeDesc m = md[i];
   const MergeOut o = mo[i];
   const int st = m.start;
+  const long zsh = (long)m.pad * sc.zmul, qsh = (long)m.pad * sc.qmul;
   DGemmProblem top;
   top.m = o.K > 0 ? m.n1 : 0;
   top.n = o.K;
   top.k = o.K1 + o.K2;
   top.tile_begin = tile_begin[2 * i];
-  top.A = sc.Qc + st + (long)st * ldq;
+  top.A = sc.Qc + (qsh + st + (long)st * ldq);
   top.lda = ldq;
-  top.B = sc.Uc + st + (long)st * ldq;
+  top.B = sc.Uc + (qsh + st + (long)st * ldq);
   top.ldb = ldq;
-  top.C = Z + st;
+  top.C = Z + (zsh + st);
   top.ldc = ldz;
   top.colmap = sc.colmap + st;
   probs[2 * i] = top;
@@ -681,11 +689,11 @@ __global__ void dc_problems_kernel(int nm, const MergeDesc *__restrict__ md, con
   bot.n = o.K;
   bot.k = o.K2 + o.K3;
   bot.tile_begin = tile_begin[2 * i + 1];
-  bot.A = sc.Qc + (st + m.n1) + (long)(st + o.K1) * ldq;
+  bot.A = sc.Qc + (qsh + (st + m.n1) + (long)(st + o.K1) * ldq);
   bot.lda = ldq;
-  bot.B = sc.Uc + (st + o.K1) + (long)st * ldq;
+  bot.B = sc.Uc + (qsh + (st + o.K1) + (long)st * ldq);
   bot.ldb = ldq;
-  bot.C = Z + st + m.n1;
+  bot.C = Z + (zsh + st + m.n1);
   bot.ldc = ldz;
   bot.colmap = sc.colmap + st;
   probs[2 * i + 1] = bot;
@@ -724,13 +732,15 @@ size_t stedc_workspace(long n) {
 // per-level tile offsets of the grouped GEMMs, copied to device memory that
 // lives as long as the context.  Only the split values (rho, sign) change per
 // solve; dc_rho_kernel writes them into the device descriptors.
-int dc_plan(evd_ctx *ctx, long n, const DcPlan **out) {
-  auto it = ctx->dc_plans.find(n);
+int dc_plan(evd_ctx *ctx, long n, int count, const DcPlan **out) {
+  const long key = n | ((long)count << 32);
+  auto it = ctx->dc_plans.find(key);
   if (it != ctx->dc_plans.end()) {
     *out = &it->second;
     return 0;
   }
-  if (ctx->capturing) return set_error(ctx, EVD_E_UNSUPPORTED, "D&C plan for n = %ld missing during capture", n);
+  if (ctx->capturing)
+    return set_error(ctx, EVD_E_UNSUPPORTED, "D&C plan for %d x n = %ld missing during capture", count, n);
   std::vector<TreeNode> nodes;
   nodes.reserve(2 * n);
   build_tree(nodes, 0, (int)n);
@@ -749,13 +759,14 @@ int dc_plan(evd_ctx *ctx, long n, const DcPlan **out) {
   for (int h = 1; h <= H; h++) {
     plan.level_off[h] = (int)all.size();
     int t = 0;
+    for (int q = 0; q < count; q++)
     for (int id : by_height[h]) {
       const TreeNode &nd = nodes[id];
       MergeDesc m;
-      m.start = nd.start;
+      m.start = (int)(q * n) + nd.start;
       m.n1 = nodes[nd.left].size;
       m.n2 = nodes[nd.right].size;
-      m.pad = 0;
+      m.pad = q;
       m.rho = 0.0;
       m.sgn = 1.0;
       all.push_back(m);
@@ -770,13 +781,14 @@ int dc_plan(evd_ctx *ctx, long n, const DcPlan **out) {
   }
   plan.level_off[H + 1] = (int)all.size();
   plan.nmerge = (int)all.size();
-  std::vector<int> split(n, 0);
+  const long nt = n * count;
+  std::vector<int> split(nt, 0);
   for (const MergeDesc &m : all) split[m.start + m.n1] = 1;
-  const size_t b_md = align256(sizeof(MergeDesc) * all.size()), b_split = align256(sizeof(int) * n),
+  const size_t b_md = align256(sizeof(MergeDesc) * all.size()), b_split = align256(sizeof(int) * nt),
                b_tiles = align256(sizeof(int) * tiles.size());
   std::vector<char> host(b_md + b_split + b_tiles, 0);
   memcpy(host.data(), all.data(), sizeof(MergeDesc) * all.size());
-  memcpy(host.data() + b_md, split.data(), sizeof(int) * n);
+  memcpy(host.data() + b_md, split.data(), sizeof(int) * nt);
   memcpy(host.data() + b_md + b_split, tiles.data(), sizeof(int) * tiles.size());
   EVD_HIP(ctx, hipMalloc(&plan.dev, host.size()));
   hipError_t e = hipMemcpy(plan.dev, host.data(), host.size(), hipMemcpyHostToDevice);
@@ -784,36 +796,61 @@ int dc_plan(evd_ctx *ctx, long n, const DcPlan **out) {
     hipFree(plan.dev);
     return set_error(ctx, EVD_E_HIP, "D&C plan upload failed: %s", hipGetErrorString(e));
   }
-  auto ins = ctx->dc_plans.emplace(n, std::move(plan));
+  auto ins = ctx->dc_plans.emplace(key, std::move(plan));
   *out = &ins.first->second;
   return 0;
 }
 
+int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double *Z, long ldz, long zs);
+
 int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
-  if (n <= 0) return 0;
+  return stedc_batched(ctx, 1, n, D, E, Z, ldz, 0);
+}
+
+size_t stedc_batched_workspace(int count, long n) {
+  const int per = (int)std::max<long>(1, 65535 / ((n + 1) / 2));
+  return stedc_workspace(n) * (size_t)std::min(count, per);
+}
+
+// `count` same-size tridiagonals (D, E of problem q at q n, Z at q zs): every
+// merge level of all problems is one set of launches.  Each problem's merges
+// and their arithmetic are those of its single solve.
+int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double *Z, long ldz, long zs) {
+  if (n <= 0 || count <= 0) return 0;
   if (n == 1) {
-    EVD_TRY(set_identity_f64(ctx, Z, 1, ldz));
+    for (int q = 0; q < count; q++) EVD_TRY(set_identity_f64(ctx, Z + q * zs, 1, ldz));
+    return 0;
+  }
+  // the merges of one level are a launch dimension (at most 65535)
+  const int per = (int)std::max<long>(1, 65535 / ((n + 1) / 2));
+  if (count > per) {
+    for (int q0 = 0; q0 < count; q0 += per) {
+      const int c = std::min(per, count - q0);
+      EVD_TRY(stedc_batched(ctx, c, n, D + q0 * n, E + q0 * n, Z + q0 * zs, ldz, zs));
+    }
     return 0;
   }
   ArenaFrame frame(ctx);
   DCScratch sc;
-  const long ldq = n;
-  sc.Qc = arena_alloc_n<double>(ctx, (size_t)n * n);
-  sc.Uc = arena_alloc_n<double>(ctx, (size_t)n * n);
+  const long ldq = n, nt = n * count;
+  sc.Qc = arena_alloc_n<double>(ctx, (size_t)n * n * count);
+  sc.Uc = arena_alloc_n<double>(ctx, (size_t)n * n * count);
   double **dbufs[] = {&sc.ds, &sc.zs, &sc.nd_d, &sc.nd_z, &sc.df_d, &sc.rot_c, &sc.rot_s, &sc.tau, &sc.lam, &sc.zhat};
-  for (auto p : dbufs) *p = arena_alloc_n<double>(ctx, (size_t)n);
+  for (auto p : dbufs) *p = arena_alloc_n<double>(ctx, (size_t)nt);
   int **ibufs[] = {&sc.cs, &sc.nd_col, &sc.qpos, &sc.df_col, &sc.rot_p, &sc.rot_n, &sc.types, &sc.orig, &sc.colmap, &sc.dmap};
-  for (auto p : ibufs) *p = arena_alloc_n<int>(ctx, (size_t)n);
+  for (auto p : ibufs) *p = arena_alloc_n<int>(ctx, (size_t)nt);
   if (!sc.Qc || !sc.Uc || !sc.zhat || !sc.dmap) return EVD_E_NOMEM;
+  sc.zmul = zs - n * (1 + ldz);
+  sc.qmul = -n;
 
   const DcPlan *plan = nullptr;
-  EVD_TRY(dc_plan(ctx, n, &plan));
+  EVD_TRY(dc_plan(ctx, n, count, &plan));
   const int nmerge = plan->nmerge, H = plan->H;
   char *pdev = static_cast<char *>(plan->dev);
   MergeDesc *dmd = reinterpret_cast<MergeDesc *>(pdev);
   const int *dsplit = reinterpret_cast<const int *>(pdev + align256(sizeof(MergeDesc) * nmerge));
   const int *dtiles = reinterpret_cast<const int *>(pdev + align256(sizeof(MergeDesc) * nmerge) +
-                                                    align256(sizeof(int) * n));
+                                                    align256(sizeof(int) * nt));
   MergeOut *dmo = static_cast<MergeOut *>(arena_alloc(ctx, sizeof(MergeOut) * nmerge));
   DGemmProblem *dprob = static_cast<DGemmProblem *>(arena_alloc(ctx, sizeof(DGemmProblem) * 2 * nmerge));
   if (!dmo || !dprob) return EVD_E_NOMEM;
@@ -822,9 +859,9 @@ int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
   // split values and signs, tearing of every split, then Q = I
   dc_rho_kernel<<<(unsigned)std::min((nmerge + 255) / 256, 1024), 256, 0, ctx->stream>>>(nmerge, dmd, E);
   EVD_LAUNCHED(ctx);
-  dc_tear_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, dsplit, E, D);
+  dc_tear_kernel<<<(unsigned)std::min<long>((nt + 255) / 256, 1024), 256, 0, ctx->stream>>>(nt, dsplit, E, D);
   EVD_LAUNCHED(ctx);
-  EVD_TRY(set_identity_f64(ctx, Z, n, ldz));
+  for (int q = 0; q < count; q++) EVD_TRY(set_identity_f64(ctx, Z + q * zs, n, ldz));
 
   // Merges level by level (by height).  Launch shapes come from the largest
   // merge of the level; every kernel bounds its work by the deflation

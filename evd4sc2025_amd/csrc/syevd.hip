@@ -23,6 +23,8 @@ int band_pack(evd_ctx *ctx, long n, long b, const float *A, long lda, double *ba
 int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E);
 int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz);
 size_t stedc_workspace(long n);
+int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double *Z, long ldz, long zs);
+size_t stedc_batched_workspace(int count, long n);
 size_t bc_back_pack_bytes(long n, long b);
 size_t bc_back_wy_bytes(long n);
 bool tune_lookup(long n, long b, int precision, int *nb, Tuning *t);
@@ -252,8 +254,9 @@ size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, in
   size_t s = per * count + 4 * 256;
   size_t later = std::max(sy2sb_batched_workspace(count, n, b, nb), sb2st_batched_workspace(count, n) + ((size_t)1 << 20));
   if (vec) {
-    s += align256((size_t)n * n * sizeof(double));
+    s += align256((size_t)n * n * count * sizeof(double));
     size_t back = syevd_workspace(n, b, nb, jobz, precision);  // per-problem back transformation and D&C bound
+    back = std::max(back, stedc_batched_workspace(count, n) + ((size_t)1 << 20));
     // batched SBR back transformation: merge products, block products, partials
     back = std::max(back, sy2sb_batched_workspace(count, n, b, nb) +
                               (align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
@@ -281,7 +284,7 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   float *U = arena_alloc_n<float>(ctx, (size_t)u_stride * count);
   double *D = arena_alloc_n<double>(ctx, (size_t)n * count);
   double *E = arena_alloc_n<double>(ctx, (size_t)e_stride * count);
-  double *Zt = vec ? arena_alloc_n<double>(ctx, (size_t)n * n) : nullptr;
+  double *Zt = vec ? arena_alloc_n<double>(ctx, (size_t)n * n * count) : nullptr;
   if (!band || !U || !D || !E || (vec && !Zt)) return EVD_E_NOMEM;
 
   for (int q = 0; q < count; q++) {
@@ -296,12 +299,12 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, n, E, e_stride));
   stage_mark(ctx, 2);
   if (vec) {
+    EVD_TRY(stedc_batched(ctx, count, n, D, E, Zt, n, n * n));
     for (int q = 0; q < count; q++) {
-      EVD_TRY(stedc(ctx, n, D + q * n, E + q * e_stride, Zt, n));
       copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D + q * n,
                                                                                                w + q * w_stride);
       EVD_LAUNCHED(ctx);
-      f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt, n, Z + q * z_stride, ldz);
+      f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt + q * n * n, n, Z + q * z_stride, ldz);
       EVD_LAUNCHED(ctx);
     }
     stage_mark(ctx, 3);
