@@ -82,6 +82,8 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
   Y += blockIdx.y * s_y;
   W += blockIdx.y * s_w;
   if (Aout) Aout += blockIdx.y * s_out;
+  part += blockIdx.y * (4L * G * 2 * NB);  // each problem's exchange ring and start tags
+  tags += blockIdx.y * G;
   const int k = min(m, nc);
   const long rec = 2 * NB;  // exchange record: NB partial sums, NB pivot-row entries
   __shared__ double red[4][NB];
@@ -110,11 +112,26 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
   }
   // Launch epoch: one more than the last launch's, read from a device word
   // (so a launch replayed from a captured graph still gets a fresh epoch and
-  // stale tags or sentinels of earlier launches never match).  Every group
-  // reads the word before it publishes its start tag; group 0 advances the
-  // word only after it has seen every start tag.
-  unsigned epoch = (ld_dev(epoch_word) + 1u) & 0x1FFFFFFu;
-  if (epoch == 0) epoch = 1;
+  // stale tags or sentinels of earlier launches never match).  Thread 0 of
+  // every group reads the word, waits for the value, then counts its group
+  // in (epoch_word[1]); the last group of the launch (all problems) resets
+  // the count and advances the word, after every group has read it.
+  __shared__ unsigned s_epoch;
+  if (tid == 0) {
+    unsigned e = (ld_dev(epoch_word) + 1u) & 0x1FFFFFFu;
+    if (e == 0) e = 1;
+    s_epoch = e;
+    if (G > 1) {
+      wait_mem();
+      const unsigned total = (unsigned)G * gridDim.y;
+      if (__hip_atomic_fetch_add(epoch_word + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1) {
+        st_dev(epoch_word + 1, 0u);
+        st_dev(epoch_word, e);
+      }
+    }
+  }
+  __syncthreads();
+  const unsigned epoch = s_epoch;
   if (G > 1) {
     // sentinels of steps 0 and 1 in this group's records, then a one-time
     // start barrier so nobody polls a slot before its owner marked it
@@ -141,7 +158,6 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
       }
     }
     __syncthreads();
-    if (g == 0 && tid == 0) st_dev(epoch_word, epoch);
     if (!s_ok) return;
   }
 
@@ -362,15 +378,17 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
                         long s_in = 0, long s_y = 0, long s_w = 0, long s_out = 0) {
   const long rows_per = 256L * RPT;
   int G = (int)((m + rows_per - 1) / rows_per);
-  if (count > 1 && (G != 1 || R || tau_out))
-    return set_error(ctx, EVD_E_UNSUPPORTED, "batched panels need one workgroup per problem and no R / tau output");
+  if (count > 1 && (R || tau_out)) return set_error(ctx, EVD_E_UNSUPPORTED, "batched panels write no R / tau output");
   int maxres = 0;
   EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<NB, RPT>, 256, 0, &maxres));
-  if (G > maxres) return set_error(ctx, EVD_E_UNSUPPORTED, "panel of %ld rows needs %d co-resident groups (max %d)", m, G, maxres);
+  if ((long)G * count > maxres)
+    return set_error(ctx, EVD_E_UNSUPPORTED, "%d panels of %ld rows need %ld co-resident groups (max %d)", count, m,
+                     (long)G * count, maxres);
   ArenaFrame frame(ctx);
-  double *part = arena_alloc_n<double>(ctx, (size_t)4 * G * 2 * NB);
+  double *part = arena_alloc_n<double>(ctx, (size_t)4 * G * 2 * NB * count);
   if (!part) return EVD_E_NOMEM;
-  if (G > kTagWords) return set_error(ctx, EVD_E_UNSUPPORTED, "panel needs %d exchange tags", G);
+  if ((long)G * count > kTagWords)
+    return set_error(ctx, EVD_E_UNSUPPORTED, "panels need %ld exchange tags", (long)G * count);
   unsigned *tags = ctx->tags;
   unsigned *epoch_word = ctx->ctrl + kCtrlEpoch;
   unsigned *err = ctx->ctrl + kCtrlError;
@@ -383,8 +401,8 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
     EVD_HIP(ctx, hipLaunchKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G, count), dim3(256), args, 0,
                                  ctx->stream));
   else
-    EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G), dim3(256), args, 0,
-                                            ctx->stream));
+    EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G, count), dim3(256), args,
+                                            0, ctx->stream));
   return 0;
 }
 
@@ -398,14 +416,22 @@ int panel_qr(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, 
 }
 
 // Same-size panels of `count` problems (A, Y, W of problem q at q times the
-// strides; the factored block is written back into A).  Panels of one
-// workgroup (m <= 512 rows for 32 columns, 256 for 64) share one launch; taller
-// panels run per problem.  Each problem's arithmetic is that of panel_qr.
+// strides; the factored block is written back into A) in one launch, grid
+// (G, count), when all G * count workgroups can be co-resident; else per
+// problem.  Each problem's arithmetic is that of panel_qr.
 int panel_qr_batched(evd_ctx *ctx, int count, long m, long nc, float *A, long lda, long sA, float *Y, long ldy,
                      long sY, float *W, long ldw, long sW) {
   if (m <= 0 || nc <= 0 || count <= 0) return 0;
   const long rows_per = nc <= 32 ? 512 : 256;
-  if (count > 1 && m <= rows_per) {
+  const long G = (m + rows_per - 1) / rows_per;
+  int maxres = 0;
+  if (count > 1 && G > 1) {
+    if (nc <= 32)
+      EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<32, 2>, 256, 0, &maxres));
+    else
+      EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<64, 1>, 256, 0, &maxres));
+  }
+  if (count > 1 && nc <= 64 && (G == 1 || (G * count <= maxres && G * count <= kTagWords))) {
     if (nc <= 32)
       return launch_panel<32, 2>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA, sY, sW,
                                  sA);
