@@ -95,11 +95,19 @@ int sy2sb(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, l
 // strides): the panel loop of sy2sb with every kernel launched once for all
 // problems.  GEMM shapes and K splits are chosen as for one problem
 // (ctx->shape_div), so each problem's result equals its sy2sb bit for bit.
+// Split-K partial buffer of one problem's largest product in the batched SBR
+// and its back transformation (floats): outputs are at most n x max(n, 1024);
+// a K split multiplies the output by at most min(k / 128, 512 / tiles + 1)
+// with tiles of at most 128 x 128 (512 = the split target), i.e. by at most
+// n / 128 + 1 chunks, and adds at most 512 * 128 * 128 words beyond one copy.
+size_t gemm_partial_words(long n) {
+  const size_t out = (size_t)n * std::max<long>(n, 1024);
+  return std::min((size_t)512 * 128 * 128 + out, (size_t)(n / 128 + 1) * out);
+}
+
 size_t sy2sb_batched_workspace(int count, long n, long b, long nb) {
   size_t per = align256((size_t)n * nb * sizeof(float)) + align256((size_t)nb * b * sizeof(float)) * 2;
-  // split-K partials: the K split of a skinny product stays below n / 128
-  // chunks, so one problem's partials stay below 2 n^2 + (n / 64 + 1) nb b words
-  size_t part = (size_t)2 * n * n * sizeof(float) + (size_t)(n / 64 + 1) * nb * b * sizeof(float);
+  size_t part = align256(gemm_partial_words(n) * sizeof(float));
   return (per + part) * (size_t)count + ((size_t)8 << 20);
 }
 
