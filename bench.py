@@ -179,6 +179,39 @@ def captured_bench(E, torch, reps: int = 20):
     return out
 
 
+def batched_form_bench(E, torch, count: int = 512, seed: int = 1):
+    """Batched form: `count` symmetric FP32 matrices, n log-uniform in
+    [64, 2048] (seed 1), uniform[-1,1] entries symmetrised, half with
+    eigenvectors: one evd_syevd_mixed_f32 call (size buckets, one batched
+    solve per bucket and jobz) against the same problems solved one at a time
+    with evd_syevd_f32; both with inputs resident in HBM."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(64), np.log(2048), size=count)).round()]
+    jobz = ["V" if i % 2 == 0 else "N" for i in range(count)]
+    ctx = E.Context(0)
+    try:
+        mats = [E.gen_symmetric(ctx, n, 5000 + i, 0) for i, n in enumerate(sizes)]
+        E.syevd_mixed(ctx, mats[:16], jobz[:16])  # warm-up: arena, plans
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E.syevd_mixed(ctx, mats, jobz)
+        torch.cuda.synchronize()
+        t_mixed = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for M, j in zip(mats, jobz):
+            E.syevd(ctx, M, b=32, nb=256 if M.shape[0] < 4096 else 1024, jobz=j)
+        torch.cuda.synchronize()
+        t_single = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    return {"matrices": count, "sizes": f"log-uniform [64, 2048], seed {seed}, half jobz=V",
+            "bucketed_s": t_mixed, "bucketed_solves_per_s": count / t_mixed,
+            "one_at_a_time_s": t_single, "one_at_a_time_solves_per_s": count / t_single,
+            "speedup": t_single / t_mixed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +225,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP16/BF16 split) runs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--no-captured", action="store_true", help="skip the eager-vs-graph small-solve sample")
+    ap.add_argument("--no-batched", action="store_true", help="skip the mixed-size batched sample")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--cpu-n", type=int, default=4096)
     args = ap.parse_args()
@@ -308,6 +342,10 @@ def main():
     if not args.no_captured:
         captured = captured_bench(E, torch)
         print(f"[bench] captured: {captured}", file=sys.stderr, flush=True)
+    batched = None
+    if not args.no_batched:
+        batched = batched_form_bench(E, torch)
+        print(f"[bench] batched form: {batched}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     line = {
         "metric": "EVD s & TFLOP/s at n=16384 FP32 b=32; mixed-size solves/s + p99 latency",
@@ -334,6 +372,7 @@ def main():
         "mixed_precision": mixed or None,
         "serving": serving,
         "captured": captured,
+        "batched_form": batched,
     }
     print(json.dumps(line))
 

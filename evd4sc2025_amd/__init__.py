@@ -22,7 +22,7 @@ __all__ = [
     "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "hou_tsqr_panel", "tc_ozimmu_syr2k", "band_pack",
     "sb2st", "stedc", "computerWYFromSlide", "computerQFromWY", "my_BC_back_trans_v8_10_noBandU", "bc_back",
     "sbr_back", "checkOrthogonality", "backward_error", "sgemm", "dgemm", "gen_symmetric", "gen_spectrum",
-    "STAGES", "autotune", "syevd_batched", "SolveGraph", "BatchedSolveGraph", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
+    "STAGES", "autotune", "syevd_batched", "syevd_mixed", "SolveGraph", "BatchedSolveGraph", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
 ]
 
 # GEMM precision of SBR and the back transformation (evd_ctx_set_precision)
@@ -369,6 +369,29 @@ def syevd(ctx: Context, A, b: int = 32, nb: int = 1024, jobz: str = "V", overwri
     ctx.call("evd_syevd_f32", _ch(jobz), n, b, nb, C.c_void_p(pa), lda, C.c_void_p(w.data_ptr()), C.c_void_p(pz),
              ldz)
     return EigResult(w[:n], Z, ctx.stage_ms())
+
+
+def syevd_mixed(ctx: Context, mats, jobz="V", b: int = 32):
+    """Problems of different orders in one call (evd_syevd_mixed_f32): each is
+    embedded in the next bucket order and every (bucket, jobz) group is one
+    batched solve.  `mats`: list of square CUDA tensors (lower triangles
+    read); `jobz`: one flag or a list.  Returns a list of EigResult."""
+    torch = _torch()
+    count = len(mats)
+    jz = [jobz] * count if isinstance(jobz, str) else list(jobz)
+    cms = [to_colmajor(M) for M in mats]
+    ws = [torch.empty(M.shape[0], dtype=torch.float64, device=M.device) for M in mats]
+    Zs = [colmajor(M.shape[0], M.shape[0], device=M.device) if j == "V" else None for M, j in zip(mats, jz)]
+    L = C.c_long * max(count, 1)
+    Pv = C.c_void_p * max(count, 1)
+    n = L(*[M.shape[0] for M in mats])
+    lda = L(*[_cm(A)[1] for A in cms])
+    ldz = L(*[(_cm(Z)[1] if Z is not None else 1) for Z in Zs])
+    pa = Pv(*[_cm(A)[0] for A in cms])
+    pw = Pv(*[w.data_ptr() for w in ws])
+    pz = Pv(*[(_cm(Z)[0] if Z is not None else 0) for Z in Zs])
+    ctx.call("evd_syevd_mixed_f32", count, n, pa, lda, pw, pz, ldz, "".join(jz).encode(), b)
+    return [EigResult(w, Z, {}) for w, Z in zip(ws, Zs)]
 
 
 class SolveGraph:

@@ -51,6 +51,8 @@ SIGNATURES = {
     "evd_syevd_batched_f32": (_i, [_vp, _ch, _i, _l, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l, _l]),
     "evd_syevd_batched_workspace_f32": (_sz, [_i, _l, _l, _l, _ch, _i]),
     "evd_sb2st_batched_f64": (_i, [_vp, _i, _l, _l, _vp, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l]),
+    "evd_syevd_mixed_f32": (_i, [_vp, _i, C.POINTER(_l), C.POINTER(_vp), C.POINTER(_l), C.POINTER(_vp),
+                                 C.POINTER(_vp), C.POINTER(_l), C.c_char_p, _l]),
     "evd_graph_create": (_i, [_vp, _ch, _l, _l, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_vp)]),
     "evd_graph_create_batched": (_i, [_vp, _ch, _i, _l, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l, _l, C.POINTER(_vp)]),
     "evd_graph_launch": (_i, [_vp]),

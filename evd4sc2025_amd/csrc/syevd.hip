@@ -5,7 +5,10 @@
 //
 // Differences from the reference driver, by design:
 //  * the tridiagonal stage runs on the GPU (FP64 divide and conquer) instead
-//    of host MKL; eigenvalues-only requests use Sturm-count bisection;
+//    of host MKL, for eigenvalues-only requests too (their tridiagonal
+//    eigenvectors stay internal: the D&C's merges need them, and it runs
+//    every eigenvalue in parallel where a Sturm-count bisection is a serial
+//    chain of n steps per eigenvalue);
 //  * eigenvectors are formed as Z = Q_SBR (Q_BC Z_T) by applying the
 //    reflectors to Z_T directly (4 n^3 flops) instead of forming Q and a
 //    final GEMM (6 n^3 in the reference);
@@ -44,82 +47,6 @@ int sbr_back_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float 
                      long ldw, const float *Y, long ldy, long wcols);
 
 static long wy_cols(long n) { return (n + 1023) / 1024 * 1024; }
-
-// ------------------------------------------------------------- bisection
-// i-th smallest eigenvalue of the symmetric tridiagonal (D, E) by Sturm
-// counts (negative pivots of the LDL^T factorisation of T - x I).
-// Gershgorin interval [gl, gu] (widened by a few ulps of its span) and the
-// pivot floor of the Sturm counts, bnd = {gl, gu, pivmin}: one workgroup,
-// min/max reductions (exact in any order).
-__global__ __launch_bounds__(256) void gershgorin_kernel(int n, const double *__restrict__ D,
-                                                         const double *__restrict__ E, double *__restrict__ bnd) {
-  __shared__ double s_lo[256], s_hi[256], s_e2[256];
-  double lo = DBL_MAX, hi = -DBL_MAX, e2 = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    const double r = (i > 0 ? fabs(E[i - 1]) : 0.0) + (i < n - 1 ? fabs(E[i]) : 0.0);
-    lo = fmin(lo, D[i] - r);
-    hi = fmax(hi, D[i] + r);
-    if (i < n - 1) e2 = fmax(e2, E[i] * E[i]);
-  }
-  s_lo[threadIdx.x] = lo;
-  s_hi[threadIdx.x] = hi;
-  s_e2[threadIdx.x] = e2;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      s_lo[threadIdx.x] = fmin(s_lo[threadIdx.x], s_lo[threadIdx.x + o]);
-      s_hi[threadIdx.x] = fmax(s_hi[threadIdx.x], s_hi[threadIdx.x + o]);
-      s_e2[threadIdx.x] = fmax(s_e2[threadIdx.x], s_e2[threadIdx.x + o]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double gl = s_lo[0], gu = s_hi[0];
-    const double span = fmax(gu - gl, DBL_MIN);
-    const double pad = __dadd_rn(__dmul_rn(2.0 * DBL_EPSILON, span), DBL_MIN);
-    bnd[0] = gl - pad;
-    bnd[1] = gu + pad;
-    bnd[2] = fmax(DBL_MIN, __dmul_rn(DBL_MIN, s_e2[0]));
-  }
-}
-
-__global__ void sturm_bisect_kernel(int n, const double *__restrict__ D, const double *__restrict__ E,
-                                    const double *__restrict__ bnd, double *__restrict__ w) {
-  const double gl = bnd[0], gu = bnd[1], pivmin = bnd[2];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double lo = gl, hi = gu;
-    for (int it = 0; it < 128; it++) {
-      const double x = 0.5 * (lo + hi);
-      if (x <= lo || x >= hi) break;
-      int cnt = 0;
-      double q = D[0] - x;
-      if (fabs(q) < pivmin) q = -pivmin;
-      cnt += (q < 0.0);
-      for (int k = 1; k < n; k++) {
-        const double e = E[k - 1];
-        q = (D[k] - x) - e * e / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        cnt += (q < 0.0);
-      }
-      if (cnt > i)
-        hi = x;
-      else
-        lo = x;
-    }
-    w[i] = 0.5 * (lo + hi);
-  }
-}
-
-static int sterf_bisect(evd_ctx *ctx, long n, const double *D, const double *E, double *w) {
-  ArenaFrame frame(ctx);
-  double *bnd = arena_alloc_n<double>(ctx, 3);
-  if (!bnd) return EVD_E_NOMEM;
-  gershgorin_kernel<<<1, 256, 0, ctx->stream>>>((int)n, D, E, bnd);
-  EVD_LAUNCHED(ctx);
-  sturm_bisect_kernel<<<(unsigned)((n + 127) / 128), 128, 0, ctx->stream>>>((int)n, D, E, bnd, w);
-  EVD_LAUNCHED(ctx);
-  return 0;
-}
 
 // ------------------------------------------------------------- helpers
 __global__ void f64_to_f32_kernel(long rows, long cols, const double *__restrict__ s, long lds, float *__restrict__ d,
@@ -167,8 +94,11 @@ size_t syevd_workspace(long n, long b, long nb, char jobz, int precision) {
   s += align256((size_t)(n + b) * n * sizeof(float));         // U
   s += align256((size_t)n * sizeof(double)) * 2;              // D, E
   size_t later = sy2sb_workspace(n, b, nb);
+  // the tridiagonal stage is the D&C for both jobz (eigenvalues only: its
+  // vectors are internal)
+  s += align256((size_t)n * n * sizeof(double));  // Z_T
+  later = std::max(later, stedc_workspace(n));
   if (jobz == 'V' || jobz == 'v') {
-    s += align256((size_t)n * n * sizeof(double));  // Z_T
     size_t dc = stedc_workspace(n);
     size_t back = align256((size_t)1024 * n * sizeof(float)) + align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
                   ((size_t)64 << 20);
@@ -195,8 +125,8 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
   float *U = arena_alloc_n<float>(ctx, (size_t)ldu * n);
   double *D = arena_alloc_n<double>(ctx, (size_t)n);
   double *E = arena_alloc_n<double>(ctx, (size_t)std::max<long>(n, 1));
-  double *Zt = vec ? arena_alloc_n<double>(ctx, (size_t)n * n) : nullptr;
-  if (!W || !Y || !band || !U || !D || !E || (vec && !Zt)) return EVD_E_NOMEM;
+  double *Zt = arena_alloc_n<double>(ctx, (size_t)n * n);
+  if (!W || !Y || !band || !U || !D || !E || !Zt) return EVD_E_NOMEM;
 
   // full symmetric storage from the lower triangle, NaN/Inf screen
   symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A, lda, ctx->ctrl + kCtrlError);
@@ -208,10 +138,10 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
   stage_mark(ctx, 1);
   EVD_TRY(sb2st(ctx, n, b, band, 2 * b, U, ldu, D, E));
   stage_mark(ctx, 2);
+  EVD_TRY(stedc(ctx, n, D, E, Zt, n));
+  copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D, w);
+  EVD_LAUNCHED(ctx);
   if (vec) {
-    EVD_TRY(stedc(ctx, n, D, E, Zt, n));
-    copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D, w);
-    EVD_LAUNCHED(ctx);
     f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt, n, Z, ldz);
     EVD_LAUNCHED(ctx);
     stage_mark(ctx, 3);
@@ -220,7 +150,6 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
     EVD_TRY(sbr_back(ctx, n, b, n, Z, ldz, W, n, Y, n, wc));
     stage_mark(ctx, 5);
   } else {
-    EVD_TRY(sterf_bisect(ctx, n, D, E, w));
     stage_mark(ctx, 3);
     stage_mark(ctx, 4);
     stage_mark(ctx, 5);
@@ -253,10 +182,10 @@ size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, in
                align256((size_t)(n + b) * n * sizeof(float)) + align256((size_t)n * sizeof(double)) * 2;
   size_t s = per * count + 4 * 256;
   size_t later = std::max(sy2sb_batched_workspace(count, n, b, nb), sb2st_batched_workspace(count, n) + ((size_t)1 << 20));
+  s += align256((size_t)n * n * count * sizeof(double));  // tridiagonal eigenvectors (internal for jobz = N)
+  later = std::max(later, stedc_batched_workspace(count, n) + ((size_t)1 << 20));
   if (vec) {
-    s += align256((size_t)n * n * count * sizeof(double));
     size_t back = syevd_workspace(n, b, nb, jobz, precision);  // per-problem back transformation and D&C bound
-    back = std::max(back, stedc_batched_workspace(count, n) + ((size_t)1 << 20));
     // batched SBR back transformation: merge products, block products, partials
     back = std::max(back, sy2sb_batched_workspace(count, n, b, nb) +
                               (align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
@@ -284,8 +213,8 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   float *U = arena_alloc_n<float>(ctx, (size_t)u_stride * count);
   double *D = arena_alloc_n<double>(ctx, (size_t)n * count);
   double *E = arena_alloc_n<double>(ctx, (size_t)e_stride * count);
-  double *Zt = vec ? arena_alloc_n<double>(ctx, (size_t)n * n * count) : nullptr;
-  if (!band || !U || !D || !E || (vec && !Zt)) return EVD_E_NOMEM;
+  double *Zt = arena_alloc_n<double>(ctx, (size_t)n * n * count);
+  if (!band || !U || !D || !E || !Zt) return EVD_E_NOMEM;
 
   for (int q = 0; q < count; q++) {
     symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A + q * a_stride, lda, ctx->ctrl + kCtrlError);
@@ -298,12 +227,14 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   stage_mark(ctx, 1);
   EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, n, E, e_stride));
   stage_mark(ctx, 2);
+  EVD_TRY(stedc_batched(ctx, count, n, D, E, Zt, n, n * n));
+  for (int q = 0; q < count; q++) {
+    copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D + q * n,
+                                                                                             w + q * w_stride);
+    EVD_LAUNCHED(ctx);
+  }
   if (vec) {
-    EVD_TRY(stedc_batched(ctx, count, n, D, E, Zt, n, n * n));
     for (int q = 0; q < count; q++) {
-      copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D + q * n,
-                                                                                               w + q * w_stride);
-      EVD_LAUNCHED(ctx);
       f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt + q * n * n, n, Z + q * z_stride, ldz);
       EVD_LAUNCHED(ctx);
     }
@@ -319,7 +250,6 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
     EVD_TRY(sbr_back_batched(ctx, count, n, b, n, Z, ldz, z_stride, W, n, Y, n, wc));
     stage_mark(ctx, 5);
   } else {
-    for (int q = 0; q < count; q++) EVD_TRY(sterf_bisect(ctx, n, D + q * n, E + q * e_stride, w + q * w_stride));
     stage_mark(ctx, 3);
     stage_mark(ctx, 4);
     stage_mark(ctx, 5);

@@ -198,6 +198,18 @@ size_t evd_syevd_batched_workspace_f32(int count, long n, long b, long nb, char 
 int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride,
                           float *U, long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride);
 
+/* ---- mixed-size batches (no counterpart in the reference, which solves one
+ * matrix per process run, src/EVD/myEVD.cu:251-706) ----
+ * count problems of any orders: host arrays n[q], A[q] (device, column major,
+ * lower triangle read, not modified), lda[q], w[q], Z[q]
+ * (NULL allowed for jobz[q] = 'N'), ldz[q], jobz[q].  Problems are embedded in
+ * the next bucket order (64, 96, 128, 192, ... up to 4096) with a diagonal pad
+ * above ||A||_inf, and each (bucket, jobz) group is one batched solve; larger
+ * orders are solved alone.  Results equal an unpadded solve within rounding
+ * (not bit for bit).  Synchronises the stream. */
+int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float *const *A, const long *lda, double *const *w,
+                        float *const *Z, const long *ldz, const char *jobz, long b);
+
 /* ---- captured solves (no counterpart in the reference, which launches every
  * kernel from the host on each call) ----
  * evd_graph_create runs one evd_syevd_f32 on the given buffers (so A is
