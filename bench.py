@@ -72,7 +72,8 @@ def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 
     Gaussian-entry symmetric matrices resident in HBM.  Saturated throughput
     of one-at-a-time plain solves (one slot, no batching) and of the serving
     engine (4 concurrent slots, same-size batching), then the same Poisson
-    arrival schedule (70% of the one-at-a-time capacity) on both."""
+    arrival schedule (70% of the one-at-a-time capacity) on both; the engine
+    also in the mixed-precision (split FP16) form."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -82,14 +83,14 @@ def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 
     pristine = [E.gen_symmetric(gctx, n, 1000 + i, 1) for i, n in enumerate(sizes)]
     torch.cuda.synchronize()
 
-    def run(slots, arrivals, batching):
+    def run(slots, arrivals, batching, precision=E.PREC_FP32):
         work = []
         for p in pristine:
             w = E.colmajor(p.shape[0], p.shape[0])
             w.copy_(p)
             work.append(w)
         torch.cuda.synchronize()
-        srv = E.Server(0, slots=slots, hbm_budget=budget, batching=batching)
+        srv = E.Server(0, slots=slots, hbm_budget=budget, batching=batching, precision=precision)
         t0 = time.perf_counter()
         for i in np.argsort(arrivals, kind="stable"):
             delay = t0 + arrivals[i] - time.perf_counter()
@@ -110,11 +111,16 @@ def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 
     arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
     poi1 = run(1, arr, False)
     poi4 = run(4, arr, True)
+    sat4h = run(4, zero, True, E.PREC_FP16)
+    poi4h = run(4, arr, True, E.PREC_FP16)
     return {"requests": nreq, "mix": "n in {256: 50%, 1024: 30%, 4096: 15%, 16384: 5%}, 70% jobz=V, Gaussian "
                                      f"entries, seed {seed}, plain FP32, HBM budget {budget >> 30} GiB",
             "saturated": {"one_at_a_time": sat1, "engine_4slots_batching": sat4,
-                          "throughput_ratio": sat4["solves_per_s"] / sat1["solves_per_s"]},
-            "poisson": {"rate_per_s": rate, "one_at_a_time": poi1, "engine_4slots_batching": poi4}}
+                          "engine_4slots_batching_fp16": sat4h,
+                          "throughput_ratio": sat4["solves_per_s"] / sat1["solves_per_s"],
+                          "throughput_ratio_fp16": sat4h["solves_per_s"] / sat1["solves_per_s"]},
+            "poisson": {"rate_per_s": rate, "one_at_a_time": poi1, "engine_4slots_batching": poi4,
+                        "engine_4slots_batching_fp16": poi4h}}
 
 
 def captured_bench(E, torch, reps: int = 20):

@@ -570,7 +570,8 @@ class Server:
     admission against an HBM budget, per-request latency.  Submitted tensors
     must stay alive until the request completes (the server keeps them)."""
 
-    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 0, batching: bool = True):
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 0, batching: bool = True,
+                 precision: int = PREC_FP32):
         self._lib = lib()
         h = C.c_void_p()
         rc = self._lib.evd_server_create(device, slots, hbm_budget, C.byref(h))
@@ -578,6 +579,9 @@ class Server:
             raise EvdError("evd_server_create", rc, "")
         self._h = h
         self._lib.evd_server_set_batching(h, 1 if batching else 0)
+        rc = self._lib.evd_server_set_precision(h, precision)
+        if rc:
+            raise EvdError("evd_server_set_precision", rc, "")
         self._keep = {}
 
     def submit(self, A, b: int = 32, nb: int = 1024, jobz: str = "V"):

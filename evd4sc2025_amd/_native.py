@@ -73,6 +73,7 @@ SIGNATURES = {
     "evd_ctx_poison_arena": (_i, [_vp, C.c_uint]),
     "evd_server_create": (_i, [_i, _i, C.c_size_t, C.POINTER(_vp)]),
     "evd_server_set_batching": (_i, [_vp, _i]),
+    "evd_server_set_precision": (_i, [_vp, _i]),
     "evd_server_destroy": (_i, [_vp]),
     "evd_server_submit": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_l)]),
     "evd_server_wait": (_i, [_vp, _l]),

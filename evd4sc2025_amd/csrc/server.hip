@@ -72,6 +72,7 @@ struct evd_server {
   std::vector<char *> stage;  // per-slot staging for batched requests
   std::vector<size_t> stage_bytes;
   bool batching = true;
+  int precision = EVD_PREC_FP32;  // GEMM precision of every slot (evd_ctx_set_precision)
   std::vector<std::thread> workers;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -172,7 +173,7 @@ static void worker_main(evd_server *s, int slot) {
           for (; cnt >= 1; cnt = (cnt > 2) ? cnt / 2 : cnt - 1) {
             const size_t ws = (cnt == 1) ? h.ws
                                          : evd::syevd_batched_workspace(cnt, h.n, h.b, h.nb, h.Z ? 'V' : 'N',
-                                                                        EVD_PREC_FP32) +
+                                                                        s->precision) +
                                                batch_stage_bytes(cnt, h.n);
             need = std::max(ws, s->arena[slot]);
             if (arena_total(s) - s->arena[slot] + need <= s->budget) {
@@ -240,6 +241,16 @@ int evd_server_set_batching(evd_server *s, int on) {
   return 0;
 }
 
+int evd_server_set_precision(evd_server *s, int precision) {
+  if (!s) return -1;
+  if (precision < EVD_PREC_FP32 || precision > EVD_PREC_FP16) return -2;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->running != 0 || !s->queue.empty()) return EVD_E_UNSUPPORTED;  // only between workloads
+  for (evd_ctx *c : s->ctx) EVD_TRY(evd_ctx_set_precision(c, precision));
+  s->precision = precision;
+  return 0;
+}
+
 int evd_server_destroy(evd_server *s) {
   if (!s) return 0;
   {
@@ -266,7 +277,11 @@ int evd_server_submit(evd_server *s, long n, long b, long nb, float *A, long lda
   if (!w && n > 0) return -7;
   if (Z && ldz < std::max<long>(1, n)) return -9;
   if (!id) return -10;
-  const size_t ws = evd::syevd_workspace(n, b, nb, Z ? 'V' : 'N', EVD_PREC_FP32);
+  size_t ws;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ws = evd::syevd_workspace(n, b, nb, Z ? 'V' : 'N', s->precision);
+  }
   if (ws > s->budget) return EVD_E_NOMEM;
   {
     std::lock_guard<std::mutex> lk(s->mu);

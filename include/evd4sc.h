@@ -303,6 +303,9 @@ int evd_server_destroy(evd_server *srv);
  * that takes a request also takes up to 31 queued requests of the same
  * (n, b, nb, jobz) and solves them as one evd_syevd_batched_f32 call. */
 int evd_server_set_batching(evd_server *srv, int on); /* finishes queued requests first */
+/* GEMM precision of every slot (EVD_PREC_*, as evd_ctx_set_precision); only
+ * while no request is queued or running (else EVD_E_UNSUPPORTED). */
+int evd_server_set_precision(evd_server *srv, int precision);
 /* Non-blocking; *id receives the request id.  EVD_E_NOMEM if the request's
  * workspace alone exceeds the budget. */
 int evd_server_submit(evd_server *srv, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz,

@@ -102,3 +102,22 @@ def test_server_same_size_requests_match_single_solves():
         if jobz == "V":
             assert np.array_equal(Z, to_host(r.Z)), seed
     ctx.close()
+
+
+def test_server_fp16_precision_matches_single_fp16_solves():
+    """A server switched to the split-FP16 form (evd_server_set_precision)
+    returns, alone or batched, exactly the single FP16 solve of each request."""
+    import evd4sc2025_amd as E
+
+    reqs = [(256, "V", 40 + i) for i in range(4)] + [(600, "V", 50), (256, "N", 51)]
+    srv = E.Server(0, slots=2, precision=E.PREC_FP16)
+    res = _solve_all(srv, reqs)
+    srv.close()
+    ctx = E.Context(0)
+    ctx.set_precision(E.PREC_FP16)
+    for (n, jobz, seed), (w, Z) in zip(reqs, res):
+        r = E.syevd(ctx, to_dev(O.gen_symmetric(n, seed, 0)), b=32, nb=256, jobz=jobz)
+        assert np.array_equal(w, to_host(r.w)[:n]), seed
+        if jobz == "V":
+            assert np.array_equal(Z, to_host(r.Z)), seed
+    ctx.close()
