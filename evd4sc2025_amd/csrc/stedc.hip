@@ -74,6 +74,45 @@ __global__ void dc_tear_kernel(long n, const int *__restrict__ split, const doub
   }
 }
 
+// Normalisation of each problem's tridiagonal to max(|d_i|, |e_i|) = 1
+// before the merges and back afterwards (the scaling step of LAPACK dstedc,
+// which the reference's MKL call performs): the deflation tolerance
+// 8 eps max(|d|max, |z|max) mixes the matrix scale with the unit-vector
+// scale of z, so it is only relative to the matrix on a normalised problem.
+// One workgroup per problem (blockIdx.x); nrm[q] keeps the factor.
+__global__ __launch_bounds__(256) void dc_normalize_kernel(long n, double *__restrict__ D, double *__restrict__ E,
+                                                           double *__restrict__ nrm) {
+  __shared__ double red[256];
+  const int q = blockIdx.x;
+  double *d = D + q * n, *e = E + q * n;
+  double mx = 0.0;
+  for (long i = threadIdx.x; i < n; i += 256) {
+    mx = fmax(mx, fabs(d[i]));
+    if (i < n - 1) mx = fmax(mx, fabs(e[i]));
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double s = red[0];
+  if (threadIdx.x == 0) nrm[q] = s;
+  if (s == 0.0 || s == 1.0) return;
+  for (long i = threadIdx.x; i < n; i += 256) {
+    d[i] /= s;
+    if (i < n - 1) e[i] /= s;
+  }
+}
+
+__global__ void dc_denormalize_kernel(long n, double *__restrict__ D, const double *__restrict__ nrm) {
+  const int q = blockIdx.y;
+  const double s = nrm[q];
+  if (s == 0.0 || s == 1.0) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    D[q * n + i] *= s;
+}
+
 // |e| and sign(e) at every split, read on the device so no host round trip
 // sits between the tridiagonal stage's producer and the merges.
 __global__ void dc_rho_kernel(int nmerge, MergeDesc *__restrict__ md, const double *__restrict__ E) {
@@ -853,10 +892,13 @@ int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double 
                                                     align256(sizeof(int) * nt));
   MergeOut *dmo = static_cast<MergeOut *>(arena_alloc(ctx, sizeof(MergeOut) * nmerge));
   DGemmProblem *dprob = static_cast<DGemmProblem *>(arena_alloc(ctx, sizeof(DGemmProblem) * 2 * nmerge));
-  if (!dmo || !dprob) return EVD_E_NOMEM;
+  double *nrm = arena_alloc_n<double>(ctx, (size_t)count);
+  if (!dmo || !dprob || !nrm) return EVD_E_NOMEM;
   unsigned *err = ctx->ctrl + kCtrlError;
 
   // split values and signs, tearing of every split, then Q = I
+  dc_normalize_kernel<<<(unsigned)count, 256, 0, ctx->stream>>>(n, D, E, nrm);
+  EVD_LAUNCHED(ctx);
   dc_rho_kernel<<<(unsigned)std::min((nmerge + 255) / 256, 1024), 256, 0, ctx->stream>>>(nmerge, dmd, E);
   EVD_LAUNCHED(ctx);
   dc_tear_kernel<<<(unsigned)std::min<long>((nt + 255) / 256, 1024), 256, 0, ctx->stream>>>(nt, dsplit, E, D);
@@ -900,6 +942,9 @@ int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double 
     dc_copy_deflated_kernel<<<dim3(gd, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq, Z, ldz);
     EVD_LAUNCHED(ctx);
   }
+  dc_denormalize_kernel<<<dim3((unsigned)std::min<long>((n + 255) / 256, 64), (unsigned)count), 256, 0,
+                          ctx->stream>>>(n, D, nrm);
+  EVD_LAUNCHED(ctx);
   return 0;
 }
 

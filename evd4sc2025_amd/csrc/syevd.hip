@@ -66,10 +66,14 @@ static dim3 grid2d(long rows, long cols) {
 }
 
 // Symmetric fill of the upper triangle from the lower one + non-finite
-// screen.  A non-finite entry raises EVD_E_NONFINITE in the error word (read
-// when the call ends) and is replaced by 0, so the rest of the solve runs on
-// finite data without a host round trip after the screen.
-__global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda, unsigned *__restrict__ err) {
+// screen + max |a_ij| (as the bits of a non-negative float, which order like
+// unsigned integers) into *amax.  A non-finite entry raises EVD_E_NONFINITE
+// in the error word (read when the call ends) and is replaced by 0, so the
+// rest of the solve runs on finite data without a host round trip after the
+// screen.
+__global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda, unsigned *__restrict__ err,
+                                        unsigned *__restrict__ amax) {
+  unsigned mx = 0;
   for (long c = blockIdx.y; c < n; c += gridDim.y)
     for (long r = c + blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
       float v = A[r + c * lda];
@@ -79,7 +83,73 @@ __global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda,
         A[r + c * lda] = v;
       }
       A[c + r * lda] = v;
+      mx = max(mx, __float_as_uint(fabsf(v)));
     }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, o, 64));
+  // one atomic per wave at most, and only while it can still raise the
+  // maximum (a stale read only lets an unneeded atomic through)
+  if ((threadIdx.x & 63) == 0 && mx > __hip_atomic_load(amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(amax, mx);
+}
+
+// Scaling into the safe range (the step of LAPACK ssyevd): with
+// anrm = max |a_ij|, a matrix with 0 < anrm < rmin is scaled by rmin / anrm and
+// one with anrm > rmax by rmax / anrm (rmin = sqrt(smlnum), rmax =
+// sqrt(1 / smlnum), smlnum = 2^-126 / 2^-23), so that no FP32 product of the
+// solve underflows or overflows; the eigenvalues are divided by the factor at
+// the end and the eigenvectors need nothing.  Problem blockIdx.z (matrices at
+// a_stride); sigma[q] receives the factor (1 when no scaling is needed).
+__device__ inline float safe_scale(unsigned amax_bits) {
+  const float anrm = __uint_as_float(amax_bits);
+  const float rmin = 3.14018491736755e-16f;  // 2^-51.5
+  const float rmax = 3.18452583626289e+15f;  // 2^51.5
+  if (anrm > 0.f && anrm < rmin) return rmin / anrm;
+  if (anrm > rmax) return rmax / anrm;
+  return 1.f;
+}
+
+__global__ void scale_input_kernel(long n, float *__restrict__ A, long lda, long a_stride,
+                                   const unsigned *__restrict__ amax, double *__restrict__ sigma) {
+  const int q = blockIdx.z;
+  const float sg = safe_scale(amax[q]);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) sigma[q] = (double)sg;
+  if (sg == 1.f) return;
+  float *Aq = A + q * a_stride;
+  for (long c = blockIdx.y; c < n; c += gridDim.y)
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+      Aq[r + c * lda] *= sg;
+}
+
+__global__ void unscale_values_kernel(long n, double *__restrict__ w, long w_stride, const double *__restrict__ sigma) {
+  const int q = blockIdx.y;
+  const double sg = sigma[q];
+  if (sg == 1.0) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    w[q * w_stride + i] /= sg;
+}
+
+// screen, symmetrise and, where needed, scale `count` matrices (A at a_stride)
+static int prepare_inputs(evd_ctx *ctx, int count, long n, float *A, long lda, long a_stride, double *sigma,
+                          unsigned *amax) {
+  EVD_TRY(fill_u32(ctx, amax, (size_t)count, 0u));
+  for (int q = 0; q < count; q++) {
+    symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A + q * a_stride, lda, ctx->ctrl + kCtrlError,
+                                                                     amax + q);
+    EVD_LAUNCHED(ctx);
+  }
+  const dim3 g = grid2d(n, n);
+  scale_input_kernel<<<dim3(g.x, std::min(g.y, 1024u), (unsigned)count), 256, 0, ctx->stream>>>(n, A, lda, a_stride,
+                                                                                              amax, sigma);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
+static int unscale_values(evd_ctx *ctx, int count, long n, double *w, long w_stride, const double *sigma) {
+  unscale_values_kernel<<<dim3((unsigned)std::min<long>((n + 255) / 256, 64), (unsigned)count), 256, 0,
+                          ctx->stream>>>(n, w, w_stride, sigma);
+  EVD_LAUNCHED(ctx);
+  return 0;
 }
 
 __global__ void copy_f64_kernel(long n, const double *__restrict__ s, double *__restrict__ d) {
@@ -126,11 +196,12 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
   double *D = arena_alloc_n<double>(ctx, (size_t)n);
   double *E = arena_alloc_n<double>(ctx, (size_t)std::max<long>(n, 1));
   double *Zt = arena_alloc_n<double>(ctx, (size_t)n * n);
-  if (!W || !Y || !band || !U || !D || !E || !Zt) return EVD_E_NOMEM;
+  double *sigma = arena_alloc_n<double>(ctx, 1);
+  unsigned *amax = arena_alloc_n<unsigned>(ctx, 1);
+  if (!W || !Y || !band || !U || !D || !E || !Zt || !sigma || !amax) return EVD_E_NOMEM;
 
-  // full symmetric storage from the lower triangle, NaN/Inf screen
-  symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A, lda, ctx->ctrl + kCtrlError);
-  EVD_LAUNCHED(ctx);
+  // full symmetric storage from the lower triangle, NaN/Inf screen, scaling
+  EVD_TRY(prepare_inputs(ctx, 1, n, A, lda, 0, sigma, amax));
 
   stage_mark(ctx, 0);
   EVD_TRY(sy2sb(ctx, n, b, nb, A, lda, W, n, Y, n, wc));
@@ -141,6 +212,7 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
   EVD_TRY(stedc(ctx, n, D, E, Zt, n));
   copy_f64_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(n, D, w);
   EVD_LAUNCHED(ctx);
+  EVD_TRY(unscale_values(ctx, 1, n, w, 0, sigma));
   if (vec) {
     f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt, n, Z, ldz);
     EVD_LAUNCHED(ctx);
@@ -214,12 +286,11 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   double *D = arena_alloc_n<double>(ctx, (size_t)n * count);
   double *E = arena_alloc_n<double>(ctx, (size_t)e_stride * count);
   double *Zt = arena_alloc_n<double>(ctx, (size_t)n * n * count);
-  if (!band || !U || !D || !E || !Zt) return EVD_E_NOMEM;
+  double *sigma = arena_alloc_n<double>(ctx, (size_t)count);
+  unsigned *amax = arena_alloc_n<unsigned>(ctx, (size_t)count);
+  if (!band || !U || !D || !E || !Zt || !sigma || !amax) return EVD_E_NOMEM;
 
-  for (int q = 0; q < count; q++) {
-    symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A + q * a_stride, lda, ctx->ctrl + kCtrlError);
-    EVD_LAUNCHED(ctx);
-  }
+  EVD_TRY(prepare_inputs(ctx, count, n, A, lda, a_stride, sigma, amax));
 
   stage_mark(ctx, 0);
   EVD_TRY(sy2sb_batched(ctx, count, n, b, nb, A, lda, a_stride, W, n, sWY, Y, n, sWY, wc));
@@ -233,6 +304,7 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
                                                                                              w + q * w_stride);
     EVD_LAUNCHED(ctx);
   }
+  EVD_TRY(unscale_values(ctx, count, n, w, w_stride, sigma));
   if (vec) {
     for (int q = 0; q < count; q++) {
       f64_to_f32_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, n, Zt + q * n * n, n, Z + q * z_stride, ldz);

@@ -124,3 +124,22 @@ def test_syevd_independent_of_stale_scratch(ctx):
         r = E.syevd(ctx, to_dev(A), b=32, nb=256)
         assert np.array_equal(to_host(r.w), w0)
         assert np.array_equal(to_host(r.Z), Z0)
+
+
+@pytest.mark.parametrize("scale", [1e-20, 1e-30, 1e20, 1e30])
+def test_syevd_extreme_norms_are_scaled(ctx, scale):
+    """Matrices far outside the FP32 product range are scaled into it before
+    the solve (LAPACK ssyevd's step) and their eigenvalues scaled back."""
+    import evd4sc2025_amd as E
+
+    n = 200
+    A = O.gen_symmetric(n, 77, 0).astype(np.float64)
+    As = (A * scale).astype(np.float32)
+    w0 = np.linalg.eigvalsh(As.astype(np.float64))
+    r = E.syevd(ctx, to_dev(As), b=32, nb=64)
+    w = to_host(r.w)
+    assert np.abs(w - w0).max() <= 1e-5 * np.abs(w0).max()
+    Z = to_host(r.Z).astype(np.float64)
+    R = A @ Z - Z * (w / scale)[None, :]
+    assert np.linalg.norm(R) / (n * np.linalg.norm(A)) <= 1e-6
+    assert np.linalg.norm(Z.T @ Z - np.eye(n)) / n <= 1e-6
