@@ -25,6 +25,7 @@ struct Tuning {
   int chase_groups = 0;    // persistent workgroups of the bulge chase (0 = auto)
   int sbr_back_block = 0;  // WY block width of the SBR back transformation (0 = 1024)
   int bc_valu = 0;         // 1: BC back transformation on the FP32 VALU kernel only
+  int skinny_split = 0;    // split-K target (workgroups per CU) of long-K skinny products (0 = 4)
 };
 
 // Merge plan of the tridiagonal divide and conquer for one order n (host

@@ -280,7 +280,7 @@ int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value) {
     int lo, hi;
   } knobs[] = {{"bc_sweeps", &ctx->tune.bc_sweeps, 0, 64},          {"bc_waves", &ctx->tune.bc_waves, 0, 16},
                {"chase_groups", &ctx->tune.chase_groups, 0, 1 << 20}, {"sbr_back_block", &ctx->tune.sbr_back_block, 0, 1 << 16},
-               {"bc_valu", &ctx->tune.bc_valu, 0, 1}};
+               {"bc_valu", &ctx->tune.bc_valu, 0, 1},           {"skinny_split", &ctx->tune.skinny_split, 0, 4}};
   for (auto &kb : knobs)
     if (!strcmp(kb.name, name)) {
       if (value < kb.lo || value > kb.hi) return -3;

@@ -229,10 +229,10 @@ __global__ void splitk_reduce_kernel(int M, int N, long batches, int ksplit, con
 template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK>
 static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
                         const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
-                        long min_chunk = 256) {
+                        long min_chunk = 256, long target_per_cu = 2) {
   long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * shape_batch(ctx, batch);
   int ksplit = 1;
-  const long target = 2L * ctx->num_cus;
+  const long target = target_per_cu * ctx->num_cus;
   // split K while the tiles alone leave the chip idle, keeping each chunk at
   // least min_chunk deep (a deterministic second pass sums the partials)
   if (tiles < target && k >= 2 * min_chunk) {
@@ -270,11 +270,15 @@ static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, con
   const long target = 2L * ctx->num_cus;
   if (n <= 32 && m > 64) {
     // tall-skinny: the row tile is chosen for parallelism first (the K
-    // dimension of these products is often short), then K is split
+    // dimension of these products is often short), then K is split; the
+    // long-K products (Z = A W over the trailing matrix, HBM-bound) split
+    // to 4 workgroups per CU, which keeps more of A in flight (2 per CU:
+    // SBR 279-283 ms at n = 16384, 4: 272 ms)
     const long hb = shape_batch(ctx, batch);
     const long t256 = (m + 255) / 256 * hb, t128 = (m + 127) / 128 * hb;
     if (t256 >= target / 2 || k >= 4096)
-      return launch_sgemm<TA, TB, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+      return launch_sgemm<TA, TB, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                                                 256, ctx->tune.skinny_split > 0 ? ctx->tune.skinny_split : 4);
     if (t128 >= target / 2)
       return launch_sgemm<TA, TB, 128, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
                                                  128);

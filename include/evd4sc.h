@@ -74,9 +74,11 @@ int evd_ctx_set_budget(evd_ctx *ctx, size_t bytes);
  * back transformation. */
 int evd_ctx_set_tuning(evd_ctx *ctx, int bc_sweeps, int bc_waves, int chase_groups, int sbr_back_block);
 /* Set one tuning knob by name (0 restores the default): "bc_sweeps",
- * "bc_waves", "chase_groups", "sbr_back_block", and "bc_valu" (1: run the BC
+ * "bc_waves", "chase_groups", "sbr_back_block", "bc_valu" (1: run the BC
  * back transformation on the FP32 VALU kernel instead of the WY blocks on the
- * matrix cores).  Returns -2 for an unknown name, -3 for a value out of range.
+ * matrix cores) and "skinny_split" (1..4: split-K workgroups per CU of the
+ * long-K skinny SBR products, default 4).  Returns -2 for an unknown name, -3
+ * for a value out of range.
  * No counterpart in the reference (its tile shapes are compile-time). */
 int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value);
 int evd_ctx_set_precision(evd_ctx *ctx, int precision);
