@@ -95,6 +95,7 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
   __shared__ double gram[NB][NB + 1];  // gram[i][j] = v_i^T v_j, i < j
   __shared__ double Tt[NB][NB + 1];    // Tt[c][l] = T[l][c]
   __shared__ double gpart[(PASSG + 1) * NB];
+  __shared__ double gsum[256];
   __shared__ int s_ok;
 
   double a[RPT][NB];
@@ -253,8 +254,20 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
           }
         }
         __syncthreads();
+        {
+          // the gn records of slot s summed by 256 / NB threads (strided
+          // partial sums), then the partials in order: a short dependent
+          // chain instead of gn LDS reads in a row
+          constexpr int P = 256 / NB;
+          const int sl = tid % NB, part = tid / NB;
+          double acc = 0.0;
+          for (int gg = part; gg < gn; gg += P) acc += gpart[gg * NB + sl];
+          gsum[tid] = acc;
+        }
+        __syncthreads();
         if (tid < NB) {
-          for (int gg = 0; gg < gn; gg++) tacc += gpart[gg * NB + tid];
+#pragma unroll
+          for (int q = 0; q < 256 / NB; q++) tacc += gsum[q * NB + tid];
           if (g0 == 0) pacc = gpart[gn * NB + tid];
         }
         __syncthreads();
