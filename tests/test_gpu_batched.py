@@ -72,3 +72,25 @@ def test_syevd_batched_fp16_equals_single(ctx):
             assert np.array_equal(to_host(Z[q]), to_host(r.Z))
     finally:
         ctx.set_precision(E.PREC_FP32)
+
+
+def test_syevd_batched_large_count_chunks_the_dc():
+    """More merges per D&C level than one launch dimension holds (300 x
+    n = 512: 76800 bottom-level merges) run as chunks of problems; sampled
+    problems must still equal their single solves bit for bit."""
+    import evd4sc2025_amd as E
+
+    torch = pytest.importorskip("torch")
+    count, n = 300, 512
+    print(f"batched count={count} n={n}", flush=True)
+    ctx = E.Context(0)
+    try:
+        mats = [E.gen_symmetric(ctx, n, 9000 + q, 0) for q in range(count)]
+        w, Z = E.syevd_batched(ctx, mats, b=32, nb=256, jobz="V")
+        torch.cuda.synchronize()
+        for q in (0, 254, 255, 299):
+            r = E.syevd(ctx, mats[q], b=32, nb=256, jobz="V")
+            assert np.array_equal(to_host(w[q]), to_host(r.w)), q
+            assert np.array_equal(to_host(Z[q].contiguous()), to_host(r.Z.contiguous())), q
+    finally:
+        ctx.close()
