@@ -40,7 +40,7 @@ def test_sb2st_batched_equals_single(ctx, count, n, b):
 
 
 @pytest.mark.parametrize("count,n,b,jobz", [(5, 200, 32, "V"), (3, 513, 32, "V"), (2, 300, 64, "V"),
-                                            (4, 256, 32, "N")])
+                                            (4, 256, 32, "N"), (2, 4096, 32, "V")])
 def test_syevd_batched_equals_single(ctx, count, n, b, jobz):
     import evd4sc2025_amd as E
 
