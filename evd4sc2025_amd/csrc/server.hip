@@ -61,7 +61,7 @@ struct Done {
 
 }  // namespace
 
-constexpr long kBatchMaxN = 4096;  // orders that are batched
+constexpr long kBatchMaxN = 16384;  // orders that are batched
 constexpr int kBatchMax = 32;      // requests per batch
 
 struct evd_server {

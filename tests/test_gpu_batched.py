@@ -94,3 +94,25 @@ def test_syevd_batched_large_count_chunks_the_dc():
             assert np.array_equal(to_host(Z[q].contiguous()), to_host(r.Z.contiguous())), q
     finally:
         ctx.close()
+
+
+def test_syevd_batched_headline_size_equals_single():
+    """Two n = 16384 problems as one batch (every stage one launch for both)
+    equal their single solves bit for bit (compared on the device)."""
+    import evd4sc2025_amd as E
+
+    torch = pytest.importorskip("torch")
+    n = 16384
+    print(f"batched count=2 n={n}", flush=True)
+    ctx = E.Context(0)
+    try:
+        mats = [E.gen_symmetric(ctx, n, 700 + q, 0) for q in range(2)]
+        w, Z = E.syevd_batched(ctx, mats, b=32, nb=1024, jobz="V")
+        for q in range(2):
+            r = E.syevd(ctx, mats[q], b=32, nb=1024, jobz="V")
+            torch.cuda.synchronize()
+            assert torch.equal(w[q], r.w[:n]), q
+            assert torch.equal(Z[q], r.Z), q
+            del r
+    finally:
+        ctx.close()
