@@ -13,7 +13,9 @@ transposed view).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 from ._native import EvdError, lib
@@ -73,6 +75,23 @@ def to_colmajor(t):
     return out
 
 
+# Native handles still open when the interpreter exits are released by an
+# atexit hook, in the order graphs, servers, contexts, while the HIP runtime is
+# still up (a handle released by a finaliser during interpreter teardown could
+# run after the runtime has gone).
+_LIVE = {"graph": weakref.WeakSet(), "server": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+
+
+@atexit.register
+def _close_live_handles():
+    for kind in ("graph", "server", "ctx"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
 class Context:
     """An evd_ctx bound to a device and (by default) torch's current stream."""
 
@@ -91,6 +110,7 @@ class Context:
         # so library work is ordered with torch's copies and kernels
         self.call("evd_ctx_set_stream", C.c_void_p(stream.cuda_stream))
         self.device = device
+        _LIVE["ctx"].add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -421,6 +441,7 @@ class SolveGraph:
         ctx.call("evd_graph_create", _ch(jobz), n, b, nb, C.c_void_p(pa), lda, C.c_void_p(self.w.data_ptr()),
                  C.c_void_p(pz), ldz, C.byref(h))
         self.h = h
+        _LIVE["graph"].add(self)
 
     def launch(self):
         self.ctx.check("evd_graph_launch", self.ctx._lib.evd_graph_launch(self.h))
@@ -469,6 +490,7 @@ class BatchedSolveGraph(SolveGraph):
                  C.c_void_p(self._w.data_ptr()), n, C.c_void_p(self._Z.data_ptr() if self._Z is not None else 0), n,
                  n * n, C.byref(h))
         self.h = h
+        _LIVE["graph"].add(self)
 
     def _copy_in(self, mats):
         for q, M in enumerate(mats):
@@ -578,6 +600,7 @@ class Server:
         if rc:
             raise EvdError("evd_server_create", rc, "")
         self._h = h
+        _LIVE["server"].add(self)
         self._lib.evd_server_set_batching(h, 1 if batching else 0)
         rc = self._lib.evd_server_set_precision(h, precision)
         if rc:
