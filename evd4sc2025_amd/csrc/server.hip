@@ -16,8 +16,9 @@
 // Same-size batching: when a slot takes a request of order n <= kBatchMaxN,
 // it also takes up to kBatchMax - 1 queued requests of the same (n, b, nb,
 // jobz), stages them into strided buffers and solves them with
-// evd_syevd_batched_f32 (one persistent chase launch and one BC back
-// transformation launch for the group; bitwise the same results).
+// evd_syevd_batched_f32 (every stage one launch for the group; bitwise the
+// same results).  Large orders gain too: the group's chase, panels and D&C
+// levels are latency-bound chains that then run side by side.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>

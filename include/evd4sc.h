@@ -301,9 +301,10 @@ int evd_gen_spectrum_f32(evd_ctx *ctx, long n, unsigned long long seed, const do
 typedef struct evd_server evd_server;
 int evd_server_create(int device, int slots, size_t hbm_budget, evd_server **out);
 int evd_server_destroy(evd_server *srv);
-/* Same-size batching of queued requests of order <= 2048 (default on): a slot
- * that takes a request also takes up to 31 queued requests of the same
- * (n, b, nb, jobz) and solves them as one evd_syevd_batched_f32 call. */
+/* Same-size batching of queued requests (default on): a slot that takes a
+ * request also takes up to 31 queued requests of the same (n, b, nb, jobz)
+ * that fit the HBM budget together and solves them as one
+ * evd_syevd_batched_f32 call (bitwise the results of single solves). */
 int evd_server_set_batching(evd_server *srv, int on); /* finishes queued requests first */
 /* GEMM precision of every slot (EVD_PREC_*, as evd_ctx_set_precision); only
  * while no request is queued or running (else EVD_E_UNSUPPORTED). */
