@@ -236,8 +236,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the eigensolver runs on a single MI355X; "
-              "see the sharding notes in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU only (the eigensolver runs on a single MI355X and has no "
+              "multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
