@@ -50,7 +50,18 @@ def cpu_baseline(n: int = 4096, seed: int = 0):
     from oracle import evd_oracle as O
 
     A = O.gen_symmetric(n, seed, 0).astype(np.float64)
-    threads = os.cpu_count() or 1
+    # The threads LAPACK actually runs on: the BLAS pool's own size (capped by
+    # OMP_NUM_THREADS / OPENBLAS_NUM_THREADS), not the machine's core count.
+    threads = 0
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max((int(i.get("num_threads") or 0) for i in threadpool_info()
+                       if i.get("user_api") == "blas"), default=0)
+    except Exception:
+        threads = 0
+    if threads <= 0:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     t0 = time.perf_counter()
     sl.eigh(A, driver="evd")
     dt = time.perf_counter() - t0
