@@ -153,6 +153,8 @@ def _tridiag_cases():
     d = np.repeat(rng.uniform(-1, 1, 8), 64) + 1e-9 * rng.standard_normal(512)
     cases.append(("cluster", d, 1e-7 * rng.standard_normal(511)))
     cases.append(("glued", np.ones(400), np.where(np.arange(399) % 50 == 49, 1e-12, 1.0)))
+    # decoupled 2x2 blocks: every other off-diagonal entry is zero
+    cases.append(("blocks2x2", rng.uniform(-1, 1, 64), np.where(np.arange(63) % 2 == 0, rng.uniform(-1, 1, 63), 0.0)))
     return cases
 
 

@@ -148,6 +148,21 @@ def test_tridiagonal_oracle_against_fixtures_and_closed_forms():
     w = O.tridiag_eig(2 * np.ones(n), np.ones(n - 1), vectors=False)
     want = np.sort(2 + 2 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))
     np.testing.assert_allclose(w, want, atol=1e-13)
+    # closed form: diagonal T has its diagonal as spectrum, Z a permutation
+    rng = np.random.default_rng(3)
+    d = rng.uniform(-1, 1, 40)
+    w, Z = O.tridiag_eig(d, np.zeros(39))
+    np.testing.assert_allclose(w, np.sort(d), atol=1e-15)
+    np.testing.assert_allclose(np.abs(Z).sum(axis=0), 1.0, atol=1e-15)
+    # closed form: decoupled 2x2 blocks [[a, e], [e, c]] have
+    # lambda = (a + c)/2 -+ sqrt(((a - c)/2)^2 + e^2)
+    a, c, e = rng.uniform(-1, 1, (3, 16))
+    d2 = np.stack([a, c], axis=1).ravel()
+    e2 = np.zeros(31)
+    e2[0::2] = e
+    m, r = (a + c) / 2, np.hypot((a - c) / 2, e)
+    want = np.sort(np.concatenate([m - r, m + r]))
+    np.testing.assert_allclose(O.tridiag_eig(d2, e2, vectors=False), want, atol=1e-14)
 
 
 @pytest.mark.parametrize("n,b,nb", [(64, 32, 32), (130, 32, 64), (200, 16, 64)])
