@@ -41,6 +41,32 @@ def profiled_traffic(kernel: str):
     return None
 
 
+def golden_eigenvalues(n: int, seed: int):
+    """Host LAPACK dsyevd (FP64) eigenvalues of the uniform workload at
+    (n, seed), committed as a fixture (tests/golden/gen_golden_headline.py),
+    with ||A||_2; None when no fixture exists for this order."""
+    import numpy as np
+
+    path = os.path.join(ROOT, "tests", "golden", f"headline_n{n}_s{seed}.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    return g["w"], float(g["norm2"])
+
+
+def accuracy(E, ctx, A0, r, gold):
+    """backward error, orthogonality (FP64 on the GPU) and, when the golden
+    dsyevd eigenvalues exist, max |w - w_dsyevd| / ||A||_2."""
+    out = {"backward_error": E.backward_error(ctx, A0, r.w, r.Z), "orthogonality": E.checkOrthogonality(ctx, r.Z)}
+    if gold is not None:
+        import numpy as np
+
+        w0, nrm2 = gold
+        out["eig_err"] = float(np.abs(r.w.cpu().numpy() - w0).max() / nrm2)
+        out["eig_err_ref"] = "host LAPACK dsyevd FP64 on the same matrix, / ||A||_2"
+    return out
+
+
 def cpu_baseline(n: int = 4096, seed: int = 0):
     """Host LAPACK dsyevd (FP64, jobz=V) on an n x n sample of the same
     workload family, timed on this machine's cores."""
@@ -314,12 +340,11 @@ def main():
     }
 
     acc = None
+    gold = golden_eigenvalues(n, 0)
     if not args.no_accuracy:
         Aw.copy_(A0)
         r = E.syevd(ctx, Aw, b=b, nb=nb, jobz="V", overwrite=True)
-        acc = {"backward_error": E.backward_error(ctx, A0, r.w, r.Z), "orthogonality": E.checkOrthogonality(ctx, r.Z),
-               "trace_rel_err": float(abs(r.w.sum().item() - torch.trace(A0.double()).item()) /
-                                      max(1.0, torch.linalg.matrix_norm(A0.double()).item()))}
+        acc = accuracy(E, ctx, A0, r, gold)
 
     print(f"[bench] timed {args.steps} steps: {dt:.3f} s/EVD, stages {stage_ms}, accuracy {acc}", file=sys.stderr,
           flush=True)
@@ -345,8 +370,7 @@ def main():
             if not args.no_accuracy:
                 Aw.copy_(A0)
                 r = E.syevd(ctx, Aw, b=b, nb=nb, jobz="V", overwrite=True)
-                entry["accuracy"] = {"backward_error": E.backward_error(ctx, A0, r.w, r.Z),
-                                     "orthogonality": E.checkOrthogonality(ctx, r.Z)}
+                entry["accuracy"] = accuracy(E, ctx, A0, r, gold)
             mixed[name] = entry
             print(f"[bench] mixed {name}: {mdt:.3f} s/EVD ({dt / mdt:.2f}x), {entry.get('accuracy')}", file=sys.stderr,
                   flush=True)

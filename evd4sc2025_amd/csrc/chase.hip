@@ -352,7 +352,7 @@ extern "C" int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, do
                                      long band_stride, float *U, long ldu, long u_stride, double *D, long d_stride,
                                      double *E, long e_stride) {
   if (!ctx) return -1;
-  EVD_ARG(ctx, count >= 0, 2);
+  EVD_ARG(ctx, count >= 0 && count <= 65535, 2);  // the band extraction carries the problem in grid y
   EVD_ARG(ctx, n >= 0, 3);
   EVD_ARG(ctx, b >= 1 && b <= 64, 4);
   EVD_ARG(ctx, ldband >= 2 * b, 6);

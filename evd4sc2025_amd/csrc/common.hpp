@@ -67,6 +67,10 @@ struct evd_ctx {
   bool capturing = false;                 // inside hipGraph capture: no host round trips
   int graphs = 0;                         // live captured graphs: the arena may not move
   long shape_div = 1;                     // GEMM launches carry this many problems' products (batched SBR)
+  // host array (one entry per problem of the next batched solve, or null):
+  // the input screen takes max |a_ij| over the leading amax_rows[q] rows and
+  // columns only (mixed-size batches: the pad diagonal is left out)
+  const long *amax_rows = nullptr;
 
   bool timing = false;
   hipEvent_t ev[EVD_STAGE_COUNT + 1] = {};
@@ -171,6 +175,11 @@ int max_coresident_blocks(evd_ctx *ctx, const void *kernel, int block, size_t dy
 
 // error word readback (synchronises the stream)
 int check_device_error(evd_ctx *ctx, const char *where);
+
+// arena bytes of a solve (syevd.hip); sbr_knob = the sbr_back_block knob the
+// solve will run with (0 = default block)
+size_t syevd_workspace(long n, long b, long nb, char jobz, int precision, long sbr_knob = 0);
+size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision, long sbr_knob = 0);
 
 // evd_syevd_f32 after its argument checks (syevd.hip)
 int syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz);

@@ -15,7 +15,6 @@
 
 namespace evd {
 
-size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision);
 
 namespace {
 
@@ -180,6 +179,16 @@ extern "C" int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float
     const dim3 grid((unsigned)((N + 255) / 256), (unsigned)N, (unsigned)c);
     pad_kernel<<<grid, 256, 0, ctx->stream>>>(N, dn, dA, dlda, norm, P);
     EVD_LAUNCHED(ctx);
+    // the scaling of the input screen follows A's entries, not the pads: the
+    // split-FP16 form scales each problem to max |a_ij| in [0.5, 1), and a
+    // pad-driven scale would push A's entries toward FP16 subnormals (pads
+    // <= 2.5 ||A||_inf <= 2.5 n max |a_ij| stay inside the FP16 range, and
+    // the reduction never multiplies them with anything but exact zeros)
+    struct AmaxRows {
+      evd_ctx *c;
+      AmaxRows(evd_ctx *c_, const long *rows) : c(c_) { c->amax_rows = rows; }
+      ~AmaxRows() { c->amax_rows = nullptr; }
+    } amax_rows(ctx, hn.data());
     EVD_TRY(syevd_batched_call(ctx, g.jobz, c, N, b, nb, P, N, N * N, wP, N, ZP, N, N * N));
     unpad_kernel<<<grid, 256, 0, ctx->stream>>>(N, dn, wP, ZP, dw, dZ, dldz);
     EVD_LAUNCHED(ctx);

@@ -214,15 +214,24 @@ int sbr_apply(evd_ctx *ctx, long n, long b, long blk, long npanel_cols, long nco
   return 0;
 }
 
-int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
-             long ldy, long wcols) {
-  long blk = ctx->tune.sbr_back_block > 0 ? ctx->tune.sbr_back_block : 1024;
+// WY block width of the SBR back transformation: the knob (0 = 1024), made
+// b * 2^k (so the doubling merge lands exactly on it) and shrunk until it
+// divides the zero-padded column count wcols (a multiple of 1024, so any
+// b * 2^k <= 1024 divides it: a knob of 2048 with wcols an odd multiple of
+// 1024 becomes 1024).
+long sbr_back_blk(long knob, long b, long wcols) {
+  long blk = knob > 0 ? knob : 1024;
   while (blk > b && blk > wcols) blk /= 2;
   if (blk < b) blk = b;
-  // blk must be b * 2^k so the doubling lands exactly on it
   long bb = b;
   while (bb * 2 <= blk) bb *= 2;
-  blk = bb;
+  while (bb > b && wcols % bb != 0) bb /= 2;
+  return bb;
+}
+
+int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
+             long ldy, long wcols) {
+  const long blk = sbr_back_blk(ctx->tune.sbr_back_block, b, wcols);
   if (wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
   EVD_TRY(wy_merge_doubling(ctx, n, b, blk, wcols, W, ldw, Y, ldy));
   return sbr_apply(ctx, n, b, blk, wcols, ncols, Z, ldz, W, ldw, Y, ldy);
@@ -234,12 +243,7 @@ int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float
 int sbr_back_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float *Z, long ldz, long sZ, float *W,
                      long ldw, const float *Y, long ldy, long wcols) {
   if (count == 1) return sbr_back(ctx, n, b, ncols, Z, ldz, W, ldw, Y, ldy, wcols);
-  long blk = ctx->tune.sbr_back_block > 0 ? ctx->tune.sbr_back_block : 1024;
-  while (blk > b && blk > wcols) blk /= 2;
-  if (blk < b) blk = b;
-  long bb = b;
-  while (bb * 2 <= blk) bb *= 2;
-  blk = bb;
+  const long blk = sbr_back_blk(ctx->tune.sbr_back_block, b, wcols);
   if (wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
   ArenaFrame frame(ctx);
   struct ShapeDiv {

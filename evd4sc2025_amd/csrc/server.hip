@@ -31,8 +31,6 @@
 #include "common.hpp"
 
 namespace evd {
-size_t syevd_workspace(long n, long b, long nb, char jobz, int precision);
-size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision);
 }
 
 extern "C" int evd_syevd_batched_f32(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda,

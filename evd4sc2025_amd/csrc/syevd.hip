@@ -30,6 +30,7 @@ int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double 
 size_t stedc_batched_workspace(int count, long n);
 size_t bc_back_pack_bytes(long n, long b);
 size_t bc_back_wy_bytes(long n);
+long sbr_back_blk(long knob, long b, long wcols);
 bool tune_lookup(long n, long b, int precision, int *nb, Tuning *t);
 int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride, float *U,
                   long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride);
@@ -71,8 +72,9 @@ static dim3 grid2d(long rows, long cols) {
 // in the error word (read when the call ends) and is replaced by 0, so the
 // rest of the solve runs on finite data without a host round trip after the
 // screen.
+// Entries at rows (and columns) >= nmax do not count toward the maximum.
 __global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda, unsigned *__restrict__ err,
-                                        unsigned *__restrict__ amax) {
+                                        unsigned *__restrict__ amax, long nmax) {
   unsigned mx = 0;
   for (long c = blockIdx.y; c < n; c += gridDim.y)
     for (long r = c + blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
@@ -83,7 +85,7 @@ __global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda,
         A[r + c * lda] = v;
       }
       A[c + r * lda] = v;
-      mx = max(mx, __float_as_uint(fabsf(v)));
+      if (r < nmax) mx = max(mx, __float_as_uint(fabsf(v)));
     }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, o, 64));
@@ -100,8 +102,22 @@ __global__ void symmetrize_check_kernel(long n, float *__restrict__ A, long lda,
 // solve underflows or overflows; the eigenvalues are divided by the factor at
 // the end and the eigenvectors need nothing.  Problem blockIdx.z (matrices at
 // a_stride); sigma[q] receives the factor (1 when no scaling is needed).
-__device__ inline float safe_scale(unsigned amax_bits) {
+//
+// The split-FP16 form (unit16 = true) instead scales every nonzero matrix by
+// the power of two that brings max |a_ij| into [0.5, 1): FP16 holds 5
+// exponent bits, so an unscaled matrix with small entries would put the lo
+// halves of the split operands into FP16 subnormals (max |a_ij| = 1e-3 keeps
+// only ~3e-5 relative accuracy) and a large one would overflow the products
+// (|A W| reaches ~0.6 sqrt(n) max |a_ij|, above 65504 for max |a_ij| = 1e3
+// at n = 16384).  A power of two changes no mantissa bit.
+__device__ inline float safe_scale(unsigned amax_bits, bool unit16) {
   const float anrm = __uint_as_float(amax_bits);
+  if (unit16) {
+    if (!(anrm > 0.f)) return 1.f;
+    int e = 0;
+    frexpf(anrm, &e);  // anrm = f 2^e, f in [0.5, 1)
+    return ldexpf(1.f, min(-e, 126));  // subnormal anrm: 2^126 still lifts it above 2^-23
+  }
   const float rmin = 3.14018491736755e-16f;  // 2^-51.5
   const float rmax = 3.18452583626289e+15f;  // 2^51.5
   if (anrm > 0.f && anrm < rmin) return rmin / anrm;
@@ -110,9 +126,9 @@ __device__ inline float safe_scale(unsigned amax_bits) {
 }
 
 __global__ void scale_input_kernel(long n, float *__restrict__ A, long lda, long a_stride,
-                                   const unsigned *__restrict__ amax, double *__restrict__ sigma) {
+                                   const unsigned *__restrict__ amax, double *__restrict__ sigma, int unit16) {
   const int q = blockIdx.z;
-  const float sg = safe_scale(amax[q]);
+  const float sg = safe_scale(amax[q], unit16 != 0);
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) sigma[q] = (double)sg;
   if (sg == 1.f) return;
   float *Aq = A + q * a_stride;
@@ -134,13 +150,14 @@ static int prepare_inputs(evd_ctx *ctx, int count, long n, float *A, long lda, l
                           unsigned *amax) {
   EVD_TRY(fill_u32(ctx, amax, (size_t)count, 0u));
   for (int q = 0; q < count; q++) {
+    const long nmax = ctx->amax_rows ? std::min(n, ctx->amax_rows[q]) : n;
     symmetrize_check_kernel<<<grid2d(n, n), 256, 0, ctx->stream>>>(n, A + q * a_stride, lda, ctx->ctrl + kCtrlError,
-                                                                     amax + q);
+                                                                     amax + q, nmax);
     EVD_LAUNCHED(ctx);
   }
   const dim3 g = grid2d(n, n);
-  scale_input_kernel<<<dim3(g.x, std::min(g.y, 1024u), (unsigned)count), 256, 0, ctx->stream>>>(n, A, lda, a_stride,
-                                                                                              amax, sigma);
+  scale_input_kernel<<<dim3(g.x, std::min(g.y, 1024u), (unsigned)count), 256, 0, ctx->stream>>>(
+      n, A, lda, a_stride, amax, sigma, ctx->precision == EVD_PREC_FP16 ? 1 : 0);
   EVD_LAUNCHED(ctx);
   return 0;
 }
@@ -156,8 +173,9 @@ __global__ void copy_f64_kernel(long n, const double *__restrict__ s, double *__
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
-size_t syevd_workspace(long n, long b, long nb, char jobz, int precision) {
+size_t syevd_workspace(long n, long b, long nb, char jobz, int precision, long sbr_knob) {
   const long wc = wy_cols(n);
+  const long blk = sbr_back_blk(sbr_knob, b, wc);
   size_t s = 0;
   s += align256((size_t)n * wc * sizeof(float)) * 2;          // W, Y
   s += align256((size_t)2 * b * n * sizeof(double));          // band
@@ -170,7 +188,7 @@ size_t syevd_workspace(long n, long b, long nb, char jobz, int precision) {
   later = std::max(later, stedc_workspace(n));
   if (jobz == 'V' || jobz == 'v') {
     size_t dc = stedc_workspace(n);
-    size_t back = align256((size_t)1024 * n * sizeof(float)) + align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
+    size_t back = align256((size_t)blk * n * sizeof(float)) + align256((size_t)(wc / 2) * blk * sizeof(float)) +
                   ((size_t)64 << 20);
     back = std::max(back, bc_back_pack_bytes(n, b) + ((size_t)1 << 20));
     if (b == 32) back = std::max(back, bc_back_wy_bytes(n) + ((size_t)1 << 20));
@@ -247,8 +265,9 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
 // interleave the problems' sweeps) and so is the BC back transformation (one
 // WY prep and one WY kernel launch, problem = grid y); SBR, the D&C and the
 // SBR back transformation run per problem in order on the context stream.
-size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision) {
+size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision, long sbr_knob) {
   const long wc = wy_cols(n);
+  const long blk = sbr_back_blk(sbr_knob, b, wc);
   const bool vec = (jobz == 'V' || jobz == 'v');
   size_t per = align256((size_t)n * wc * sizeof(float)) * 2 + align256((size_t)2 * b * n * sizeof(double)) +
                align256((size_t)(n + b) * n * sizeof(float)) + align256((size_t)n * sizeof(double)) * 2;
@@ -257,11 +276,11 @@ size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, in
   s += align256((size_t)n * n * count * sizeof(double));  // tridiagonal eigenvectors (internal for jobz = N)
   later = std::max(later, stedc_batched_workspace(count, n) + ((size_t)1 << 20));
   if (vec) {
-    size_t back = syevd_workspace(n, b, nb, jobz, precision);  // per-problem back transformation and D&C bound
+    size_t back = syevd_workspace(n, b, nb, jobz, precision, sbr_knob);  // per-problem back transformation and D&C bound
     // batched SBR back transformation: merge products, block products, partials
     back = std::max(back, sy2sb_batched_workspace(count, n, b, nb) +
-                              (align256((size_t)(wc / 2) * 1024 * sizeof(float)) +
-                               align256((size_t)1024 * n * sizeof(float))) * (size_t)count);
+                              (align256((size_t)(wc / 2) * blk * sizeof(float)) +
+                               align256((size_t)blk * n * sizeof(float))) * (size_t)count);
     if (b == 32) back = std::max(back, bc_back_wy_bytes(n) * count + ((size_t)1 << 20));
     later = std::max(later, back);
   }
@@ -420,7 +439,7 @@ int evd::syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, 
     if (!ctx->tune.sbr_back_block) ctx->tune.sbr_back_block = t.sbr_back_block;
   }
   if (nb == 0) nb = tnb > 0 ? tnb : b * std::max<long>(1, 1024 / b);
-  int s = arena_ensure(ctx, syevd_workspace(n, b, nb, jobz, ctx->precision));
+  int s = arena_ensure(ctx, syevd_workspace(n, b, nb, jobz, ctx->precision, ctx->tune.sbr_back_block));
   if (s == 0) s = syevd(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
   ctx->tune = user;
   return s;
@@ -436,7 +455,8 @@ extern "C" int evd_syevd_batched_f32(evd_ctx *ctx, char jobz, int count, long n,
   if (!ctx) return -1;
   const bool vec = (jobz == 'V' || jobz == 'v');
   EVD_ARG(ctx, vec || jobz == 'N' || jobz == 'n', 2);
-  EVD_ARG(ctx, count >= 0, 3);
+  // count <= 65535: per-problem kernels carry the problem index in grid y / z
+  EVD_ARG(ctx, count >= 0 && count <= 65535, 3);
   EVD_ARG(ctx, n >= 0 && (long)count * n < (1L << 30), 4);
   EVD_ARG(ctx, b == 32 || b == 64 || (!vec && b >= 1 && b <= 64), 5);
   EVD_ARG(ctx, nb == 0 || (nb >= b && nb % b == 0), 6);
@@ -464,7 +484,7 @@ int evd::syevd_batched_call(evd_ctx *ctx, char jobz, int count, long n, long b, 
     if (!ctx->tune.sbr_back_block) ctx->tune.sbr_back_block = t.sbr_back_block;
   }
   if (nb == 0) nb = tnb > 0 ? tnb : b * std::max<long>(1, 1024 / b);
-  int s = arena_ensure(ctx, syevd_batched_workspace(count, n, b, nb, jobz, ctx->precision));
+  int s = arena_ensure(ctx, syevd_batched_workspace(count, n, b, nb, jobz, ctx->precision, ctx->tune.sbr_back_block));
   if (s == 0) s = syevd_batched(ctx, jobz, count, n, b, nb, A, lda, a_stride, w, w_stride, Z, ldz, z_stride);
   ctx->tune = user;
   return s;

@@ -34,10 +34,14 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_library_is_gfx950_code_object():
+    """Every device code object bundled in the library targets gfx950 and
+    nothing else, and no NVIDIA fat binary or PTX section is present."""
     so = ROOT / "evd4sc2025_amd" / "libevd4sc.so"
     data = so.read_bytes()
-    assert b"gfx950" in data
-    assert b"sm_" not in data[:0]  # no CUDA binaries by construction
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa--(gfx[0-9a-z]+)", data))
+    assert targets == {b"gfx950"}, targets
+    for marker in (b"__nv_fatbin", b".nv_fatbin", b"nvptx64", b"__nv_module_id"):
+        assert marker not in data, marker
 
 
 def test_workspace_query_is_host_only():

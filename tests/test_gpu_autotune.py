@@ -49,3 +49,22 @@ def test_cached_knobs_keep_accuracy(tctx, knobs):
     tctx.use_tune_cache(False)
     _check(E, tctx, O.gen_symmetric(700, 12, 0), 32, 0)
     tctx.use_tune_cache(True)
+
+
+@pytest.mark.parametrize("n", [3000, 9000])
+def test_sbr_back_block_2048_at_odd_multiples_of_1024(tctx, n):
+    """W and Y are padded to a multiple of 1024 columns; with 3 or 9 such
+    blocks a 2048-wide SBR back-transformation block does not divide the
+    padding, so the block shrinks to 1024 (both as a knob and from the cache
+    entry of the order's bucket)."""
+    import evd4sc2025_amd as E
+
+    print(f"sbr_back_block 2048 at n={n}", flush=True)
+    E.tune_cache_put(n, 32, E.PREC_FP32, {"nb": 1024, "chase_groups": 0, "bc_valu": 0, "sbr_back_block": 2048})
+    A = O.gen_symmetric(n, 13, 0)
+    _check(E, tctx, A, 32, 0)
+    tctx.use_tune_cache(False)
+    tctx.set_knob("sbr_back_block", 2048)
+    _check(E, tctx, A, 32, 1024)
+    tctx.set_knob("sbr_back_block", 0)
+    tctx.use_tune_cache(True)

@@ -241,3 +241,16 @@ def test_hou_tsqr_panel_explicit_q(ctx, m, b):
     assert np.abs(Q @ R - P).max() < 1e-5 * np.sqrt(m)
     _, _, R0, _ = O.panel_qr(P.astype(np.float64))
     np.testing.assert_allclose(R, R0, atol=1e-4)
+
+
+@pytest.mark.parametrize("n,seed", [(64, 3), (257, 8)])
+def test_gaussian_generator_matches_oracle(ctx, n, seed):
+    """kind = 1 (Box-Muller): the device's FP64 log / cos may differ from
+    NumPy's in the last bit, so the FP32 entries must agree to one rounding
+    (relative 2^-23) and almost all of them exactly."""
+    import evd4sc2025_amd as E
+
+    A = to_host(E.gen_symmetric(ctx, n, seed, 1)).astype(np.float64)
+    B = O.gen_symmetric(n, seed, 1).astype(np.float64)
+    assert np.all(np.abs(A - B) <= 2.0 ** -23 * np.abs(B) + 1e-30)
+    assert np.mean(A == B) > 0.99

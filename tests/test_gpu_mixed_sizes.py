@@ -55,3 +55,23 @@ def test_mixed_sizes_zero_and_tiny(ctx):
     assert np.all(w0 == 0.0)
     assert np.linalg.norm(Z0.T.astype(np.float64) @ Z0 - np.eye(80)) / 80 <= 1e-6
     _check(mats[1].astype(np.float64), to_host(res[1].w), to_host(res[1].Z))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_mixed_sizes_near_largest_bucket(ctx, prec):
+    """Orders just above and below a bucket boundary of the batched form's
+    range (bucket 2048: pads of 1 to 511 rows, ||A||_inf / ||A||_2 ~ 16), at
+    the 1e-5 ||A||_2 bar, in the plain and the split-FP16 forms."""
+    import evd4sc2025_amd as E
+
+    sizes = [1537, 1800, 2047, 1600]
+    jobz = ["V", "N", "V", "N"]
+    mats = [O.gen_symmetric(n, 400 + i, 0) for i, n in enumerate(sizes)]
+    print(f"mixed sizes {sizes} {prec}", flush=True)
+    ctx.set_precision(E.PREC_FP16 if prec == "fp16" else E.PREC_FP32)
+    try:
+        res = E.syevd_mixed(ctx, [to_dev(M) for M in mats], jobz)
+    finally:
+        ctx.set_precision(E.PREC_FP32)
+    for M, r in zip(mats, res):
+        _check(M, to_host(r.w), to_host(r.Z) if r.Z is not None else None)

@@ -143,3 +143,29 @@ def test_syevd_extreme_norms_are_scaled(ctx, scale):
     R = A @ Z - Z * (w / scale)[None, :]
     assert np.linalg.norm(R) / (n * np.linalg.norm(A)) <= 1e-6
     assert np.linalg.norm(Z.T @ Z - np.eye(n)) / n <= 1e-6
+
+
+@pytest.mark.parametrize("scale", [1e-3, 1e3])
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_split_forms_at_input_scales(ctx, scale, prec):
+    """The split 16-bit forms on a scaled matrix (n = 2048, the golden
+    dsyevd input times 1e-3 or 1e3): FP16 has 5 exponent bits, so the solver
+    scales the input to a unit max-norm first; the bars are those of the
+    plain form."""
+    import evd4sc2025_amd as E
+
+    n = 2048
+    dA = E.gen_symmetric(ctx, n, 0, 0)
+    dA.mul_(scale)
+    w0 = G[f"syevd_w_n{n}_s0"] * scale
+    print(f"{prec} at input scale {scale:g}", flush=True)
+    ctx.set_precision(E.PREC_FP16 if prec == "fp16" else E.PREC_BF16)
+    try:
+        r = E.syevd(ctx, dA, b=32, nb=1024, jobz="V")
+    finally:
+        ctx.set_precision(E.PREC_FP32)
+    w = to_host(r.w)
+    assert np.all(np.isfinite(w))
+    assert np.abs(w - w0).max() <= 1e-5 * np.abs(w0).max()
+    assert E.backward_error(ctx, dA, r.w, r.Z) <= 1e-6
+    assert E.checkOrthogonality(ctx, r.Z) <= 1e-6

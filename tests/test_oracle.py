@@ -201,3 +201,44 @@ def test_planted_mistake_is_caught():
         O.panel_qr = orig
     bad = np.abs(np.linalg.eigvalsh(O.band_of(Ab2, b)) - np.linalg.eigvalsh(A)).max()
     assert good < 1e-12 and bad > 1e-3
+
+
+def test_gaussian_generator_moments():
+    """kind = 1: off-diagonal entries (x + y)/2 of two independent N(0, 1)
+    draws are N(0, 1/2); the diagonal entries are N(0, 1).  Moments and a
+    Kolmogorov-Smirnov test pin the Box-Muller mapping (a wrong scale, a
+    missing sqrt(-2 log u) factor or a uniform in place of a normal fails)."""
+    import scipy.stats as st
+
+    n = 768
+    A = O.gen_symmetric(n, 5, 1).astype(np.float64)
+    off = A[np.tril_indices(n, -1)]
+    N = off.size
+    assert abs(off.mean()) < 5.0 * np.sqrt(0.5 / N)
+    assert abs(off.var() - 0.5) < 5.0 * 0.5 * np.sqrt(2.0 / N)
+    kurt = ((off - off.mean()) ** 4).mean() / off.var() ** 2
+    assert abs(kurt - 3.0) < 0.1
+    assert st.kstest(off, "norm", args=(0.0, np.sqrt(0.5))).pvalue > 1e-3
+    assert st.kstest(np.diag(A), "norm", args=(0.0, 1.0)).pvalue > 1e-3
+    assert np.array_equal(A, A.T)
+
+
+def test_metric_definitions_known_deviation():
+    """The metrics against deviations of known size, so that a dropped
+    normalisation (the 1/n of either metric, the 1/||A||_F of the backward
+    error) or a norm other than Frobenius fails."""
+    n, dlt = 50, 1e-3
+    # Q = (1 + d) I: Q^T Q - I = (2d + d^2) I, ||.||_F = (2d + d^2) sqrt(n)
+    Q = (1.0 + dlt) * np.eye(n)
+    np.testing.assert_allclose(O.orthogonality(Q), (2 * dlt + dlt ** 2) * np.sqrt(n) / n, rtol=1e-12)
+    # a single off-diagonal unit: ||Q^T Q - I||_F^2 = 2 + 1 (two symmetric
+    # entries of 1 and one diagonal entry of 1)
+    Q = np.eye(n)
+    Q[3, 7] = 1.0
+    np.testing.assert_allclose(O.orthogonality(Q), np.sqrt(3.0) / n, rtol=1e-12)
+    # A = diag(a), Z = I, w = a + d e_k: A Z - Z diag(w) = -d e_k e_k^T
+    rng = np.random.default_rng(1)
+    a = rng.uniform(-2, 2, n)
+    w = a.copy()
+    w[11] += dlt
+    np.testing.assert_allclose(O.backward_error(np.diag(a), w, np.eye(n)), dlt / (n * np.linalg.norm(a)), rtol=1e-12)
