@@ -248,16 +248,17 @@ def band_pack(ctx: Context, A, b: int):
     return band
 
 
-def sb2st(ctx: Context, band, n: int, b: int):
-    """Bulge chasing (util/inc/bugle_chasingV10.h:21) -> D, E, U."""
+def sb2st(ctx: Context, band, n: int, b: int, noalloc: bool = False):
+    """Bulge chasing (util/inc/bugle_chasingV10.h:21) -> D, E, U.  noalloc:
+    through evd_sb2st_f64_noalloc (scratch in a fixed device array, n <= 32768)."""
     torch = _torch()
     U = colmajor(n + b, n)
     D = torch.empty(n, dtype=torch.float64, device=band.device)
     E = torch.empty(max(n - 1, 1), dtype=torch.float64, device=band.device)
     pb, ldb = _cm(band)
     pu, ldu = _cm(U)
-    ctx.call("evd_sb2st_f64", n, b, C.c_void_p(pb), ldb, C.c_void_p(pu), ldu, C.c_void_p(D.data_ptr()),
-             C.c_void_p(E.data_ptr()))
+    ctx.call("evd_sb2st_f64_noalloc" if noalloc else "evd_sb2st_f64", n, b, C.c_void_p(pb), ldb, C.c_void_p(pu), ldu,
+             C.c_void_p(D.data_ptr()), C.c_void_p(E.data_ptr()))
     return D, E[: max(n - 1, 0)], U
 
 

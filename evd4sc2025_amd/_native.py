@@ -39,6 +39,7 @@ SIGNATURES = {
     "evd_sy2sb_f32": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
     "evd_band_pack_f32": (_i, [_vp, _l, _l, _vp, _l, _vp, _l]),
     "evd_sb2st_f64": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp, _vp]),
+    "evd_sb2st_f64_noalloc": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp, _vp]),
     "evd_stedc_f64": (_i, [_vp, _l, _vp, _vp, _vp, _l]),
     "evd_wy_from_slide_f32": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l]),
     "evd_q_from_wy_f32": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp, _l]),

@@ -26,6 +26,8 @@
 // published a next-row >= p + 2b - 1 (a lag of two steps, not three).
 // Every wait is bounded by a watchdog so the grid always drains.
 #include <climits>
+#include <map>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -291,8 +293,10 @@ __global__ void band_extract_batched_kernel(int n, const double *__restrict__ ba
 
 // Bulge chasing of `count` same-size bands in one persistent launch (problem q
 // at band + q band_stride, U + q u_stride, D + q d_stride, E + q e_stride).
+// The progress words and the ticket come from the arena, or from ext_com
+// (at least sb2st_batched_workspace(count, n) bytes) when it is given.
 int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride, float *U,
-                  long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride) {
+                  long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride, int *ext_com) {
   if (n <= 0 || count <= 0) return 0;
   for (int q = 0; q < count; q++) EVD_TRY(fill_f32(ctx, U + (long)q * u_stride, ldu, n, ldu, 0.f));
   if (n > 2) {
@@ -300,7 +304,7 @@ int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ld
     // progress words com[q * n * kComStride + s * kComStride], then the ticket
     const long com_stride = n * kComStride;
     const size_t words = (size_t)count * com_stride + 1;
-    int *com = arena_alloc_n<int>(ctx, words);
+    int *com = ext_com ? ext_com : arena_alloc_n<int>(ctx, words);
     if (!com) return EVD_E_NOMEM;
     unsigned *ticket = reinterpret_cast<unsigned *>(com + (size_t)count * com_stride);
     EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), words, 0u));
@@ -320,7 +324,39 @@ int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ld
 size_t sb2st_batched_workspace(int count, long n) { return align256(((size_t)count * n * kComStride + 1) * sizeof(int)); }
 
 int sb2st(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E) {
-  return sb2st_batched(ctx, 1, n, b, band, ldband, 0, U, ldu, 0, D, 0, E, 0);
+  return sb2st_batched(ctx, 1, n, b, band, ldband, 0, U, ldu, 0, D, 0, E, 0, nullptr);
+}
+
+// ---- the chase without allocation: scratch in a file-scope device array.
+// Progress words (one per sweep, kComStride apart) and the sweep ticket of
+// one problem of order <= kNoAllocMaxN: 4 MiB per device, held until the
+// process ends.  Host code never passes the array's name to a kernel (it is a
+// host-side stand-in); it uses the device address that hipGetSymbolAddress
+// returns, looked up once per device with that device current.
+constexpr long kNoAllocMaxN = 32768;
+constexpr size_t kNoAllocWords = (size_t)kNoAllocMaxN * kComStride + 64;
+static __device__ __align__(256) int scratch[kNoAllocWords];
+
+static int noalloc_scratch(evd_ctx *ctx, int **out) {
+  static std::mutex mu;
+  static std::map<int, int *> ptrs;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = ptrs.find(ctx->device);
+  if (it != ptrs.end()) {
+    *out = it->second;
+    return 0;
+  }
+  int cur = -1;
+  EVD_HIP(ctx, hipGetDevice(&cur));
+  if (cur != ctx->device) EVD_HIP(ctx, hipSetDevice(ctx->device));
+  int *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  if (cur != ctx->device) EVD_HIP(ctx, hipSetDevice(cur));
+  if (e != hipSuccess || !p)
+    return set_error(ctx, EVD_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+  ptrs[ctx->device] = p;
+  *out = p;
+  return 0;
 }
 
 }  // namespace evd
@@ -348,6 +384,24 @@ extern "C" int evd_sb2st_f64(evd_ctx *ctx, long n, long b, double *band, long ld
   return check_device_error(ctx, "evd_sb2st_f64");
 }
 
+extern "C" int evd_sb2st_f64_noalloc(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu,
+                                     double *D, double *E) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, n >= 0, 2);
+  EVD_ARG(ctx, b >= 1 && b <= 64, 3);
+  EVD_ARG(ctx, ldband >= 2 * b, 5);
+  EVD_ARG(ctx, ldu >= n + b, 7);
+  if (n > kNoAllocMaxN)
+    return set_error(ctx, EVD_E_UNSUPPORTED, "evd_sb2st_f64_noalloc: n = %ld exceeds the scratch limit n <= %ld", n,
+                     kNoAllocMaxN);
+  static_assert(kNoAllocWords * sizeof(int) >= ((size_t)kNoAllocMaxN * kComStride + 1) * sizeof(int),
+                "scratch holds the progress words and the ticket of the largest order");
+  int *com = nullptr;
+  EVD_TRY(noalloc_scratch(ctx, &com));
+  EVD_TRY(sb2st_batched(ctx, 1, n, b, band, ldband, 0, U, ldu, 0, D, 0, E, 0, com));
+  return check_device_error(ctx, "evd_sb2st_f64_noalloc");
+}
+
 extern "C" int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, double *band, long ldband,
                                      long band_stride, float *U, long ldu, long u_stride, double *D, long d_stride,
                                      double *E, long e_stride) {
@@ -363,6 +417,7 @@ extern "C" int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, do
   EVD_ARG(ctx, count <= 1 || e_stride >= std::max<long>(1, n - 1), 14);
   EVD_ARG(ctx, (long)count * std::max<long>(n, 1) < (1L << 30), 2);
   if (ctx->arena_top == 0) EVD_TRY(arena_ensure(ctx, sb2st_batched_workspace(count, n) + 4096));
-  EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, d_stride, E, e_stride));
+  EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, d_stride, E, e_stride,
+                        nullptr));
   return check_device_error(ctx, "evd_sb2st_batched_f64");
 }

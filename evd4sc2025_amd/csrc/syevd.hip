@@ -33,7 +33,7 @@ size_t bc_back_wy_bytes(long n);
 long sbr_back_blk(long knob, long b, long wcols);
 bool tune_lookup(long n, long b, int precision, int *nb, Tuning *t);
 int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ldband, long band_stride, float *U,
-                  long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride);
+                  long ldu, long u_stride, double *D, long d_stride, double *E, long e_stride, int *ext_com);
 size_t sb2st_batched_workspace(int count, long n);
 bool bc_back_wy_fits(long n, long ldq);
 int bc_back_wy_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride, const float *U,
@@ -315,7 +315,7 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
   EVD_TRY(sy2sb_batched(ctx, count, n, b, nb, A, lda, a_stride, W, n, sWY, Y, n, sWY, wc));
   for (int q = 0; q < count; q++) EVD_TRY(band_pack(ctx, n, b, A + q * a_stride, lda, band + q * band_stride, ldband));
   stage_mark(ctx, 1);
-  EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, n, E, e_stride));
+  EVD_TRY(sb2st_batched(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, D, n, E, e_stride, nullptr));
   stage_mark(ctx, 2);
   EVD_TRY(stedc_batched(ctx, count, n, D, E, Zt, n, n * n));
   for (int q = 0; q < count; q++) {

@@ -141,6 +141,17 @@ int evd_band_pack_f32(evd_ctx *ctx, long n, long b, const float *A, long lda, do
  * util/kernelOther/kernelOther.cu:399-440). */
 int evd_sb2st_f64(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D, double *E);
 
+/* evd_sb2st_f64 without any device allocation, for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results (bit for bit) as evd_sb2st_f64; its scratch (one progress word per
+ * sweep and the sweep ticket) is a fixed 4 MiB array per device instead of
+ * the context arena, so n is limited to 32768 (EVD_E_UNSUPPORTED above).
+ * Calls on one device share that array: calls that overlap on one device
+ * (any contexts or streams) race and must be serialised by the caller. */
+int evd_sb2st_f64_noalloc(evd_ctx *ctx, long n, long b, double *band, long ldband, float *U, long ldu, double *D,
+                          double *E);
+
 /* Symmetric tridiagonal divide and conquer on the GPU in FP64.  On exit D
  * holds the eigenvalues in ascending order and Z (n x n) the orthonormal
  * eigenvectors (compz = 'I').  E is destroyed.
