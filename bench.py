@@ -67,97 +67,151 @@ def accuracy(E, ctx, A0, r, gold):
     return out
 
 
-def cpu_baseline(n: int = 4096, seed: int = 0):
-    """Host LAPACK dsyevd (FP64, jobz=V) on an n x n sample of the same
-    workload family, timed on this machine's cores."""
+def _mkl():
+    """The image's MKL runtime (the library the reference links for its host
+    stage), or None."""
+    import ctypes
+
+    for path in ("/opt/conda/lib/libmkl_rt.so.1", "libmkl_rt.so.1", "libmkl_rt.so"):
+        try:
+            lib = ctypes.CDLL(path)
+        except OSError:
+            continue
+        if hasattr(lib, "LAPACKE_dstedc") and hasattr(lib, "LAPACKE_dsyevd"):
+            return lib
+    return None
+
+
+def host_threads() -> int:
+    """Threads the host LAPACK runs on: OMP_NUM_THREADS when set (the GPU
+    box's CPU share), else the machine's core count."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+
+def cpu_baseline(A_host, D, Ed, full: bool):
+    """Host LAPACK on the headline workload, on this machine's cores: MKL
+    LAPACKE_dstedc on the (D, E) the GPU chase produced for the headline
+    matrix (the reference's own host stage, src/EVD/myEVD.cu:529) and, with
+    full=True, MKL LAPACKE_dsyevd (jobz=V) on the headline matrix itself;
+    without it, dsyevd on the n = 8192 matrix of the same workload family
+    (uniform, seed 0) as a bounded sample.  value = (10/3 n^3) / seconds of
+    the dsyevd run, the metric's normalisation."""
+    import ctypes
+
     import numpy as np
-    import scipy.linalg as sl
 
-    from oracle import evd_oracle as O
-
-    A = O.gen_symmetric(n, seed, 0).astype(np.float64)
-    # The threads LAPACK actually runs on: the BLAS pool's own size (capped by
-    # OMP_NUM_THREADS / OPENBLAS_NUM_THREADS), not the machine's core count.
-    threads = 0
-    try:
-        from threadpoolctl import threadpool_info
-
-        threads = max((int(i.get("num_threads") or 0) for i in threadpool_info()
-                       if i.get("user_api") == "blas"), default=0)
-    except Exception:
-        threads = 0
-    if threads <= 0:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    mkl = _mkl()
+    threads = host_threads()
+    if mkl is None:
+        return None
+    mkl.MKL_Set_Num_Threads(ctypes.c_int(threads))
+    n = D.shape[0]
+    d = np.array(D, dtype=np.float64)
+    e = np.array(Ed, dtype=np.float64)
+    Z = np.empty((n, n), dtype=np.float64, order="F")
+    dp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
     t0 = time.perf_counter()
-    sl.eigh(A, driver="evd")
-    dt = time.perf_counter() - t0
+    rc = mkl.LAPACKE_dstedc(102, ctypes.c_char(b"I"), n, dp(d), dp(e), dp(Z), n)
+    t_stedc = time.perf_counter() - t0
+    del Z
+    if rc != 0:
+        return None
+    if full:
+        A = np.array(A_host, dtype=np.float64, order="F")
+        nn = n
+    else:
+        from oracle import evd_oracle as O
+
+        nn = 8192
+        A = np.asfortranarray(O.gen_symmetric(nn, 0, 0).astype(np.float64))
+    w = np.empty(nn)
+    t0 = time.perf_counter()
+    rc = mkl.LAPACKE_dsyevd(102, ctypes.c_char(b"V"), ctypes.c_char(b"L"), nn, dp(A), nn, dp(w))
+    t_syevd = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    which = "the headline matrix" if full else f"the n={nn} matrix of the same workload (uniform[-1,1], seed 0)"
     return {
-        "value": (10.0 / 3.0) * n ** 3 / dt / 1e12,
+        "value": (10.0 / 3.0) * nn ** 3 / t_syevd / 1e12,
         "unit": "TFLOP/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"host LAPACK dsyevd (scipy/OpenBLAS, FP64, jobz=V) at n={n}, uniform seed {seed}: {dt:.2f} s "
-                  f"(same 10/3 n^3 normalisation); the reference's own CPU stage is MKL dstedc",
-        "seconds": dt,
-        "n": n,
+        "sample": f"MKL LAPACKE_dsyevd (FP64, jobz=V) on {which}: {t_syevd:.2f} s (10/3 n^3 normalisation); "
+                  f"MKL LAPACKE_dstedc on the headline (D, E) - the reference's own host stage - {t_stedc:.2f} s",
+        "dsyevd_s": t_syevd,
+        "dsyevd_n": nn,
+        "host_dstedc_s": t_stedc,
+        "host_dstedc_n": n,
     }
 
 
-def serving_bench(E, torch, nreq: int = 60, seed: int = 1, budget: int = 200 << 30):
-    """Serving form on a bounded sample: nreq requests, n drawn from
-    {256: 50%, 1024: 30%, 4096: 15%, 16384: 5%}, 70% with eigenvectors,
-    Gaussian-entry symmetric matrices resident in HBM.  Saturated throughput
-    of one-at-a-time plain solves (one slot, no batching) and of the serving
-    engine (4 concurrent slots, same-size batching), then the same Poisson
-    arrival schedule (70% of the one-at-a-time capacity) on both; the engine
-    also in the mixed-precision (split FP16) form."""
+def serving_bench(E, torch, nreq: int = 400, seed: int = 1, budget: int = 200 << 30, slots: int = 4):
+    """Serving form on a bounded sample of the request stream: nreq requests
+    in the exact proportions n = 256 (50%), 1024 (30%), 4096 (15%), 16384
+    (5%), 70% with eigenvectors, Gaussian-entry symmetric matrices resident in
+    HBM, against an HBM budget pool.  Runs: one-at-a-time plain FP32 solves
+    (one slot, no batching) saturated; the engine (`slots` slots, same-size
+    batching, largest-order-first admission) saturated in plain FP32 and in
+    the split-FP16 form; then one Poisson schedule at 70% of the
+    one-at-a-time capacity for one-at-a-time solves, and the same schedule at
+    twice that rate for the engine: the target is 2x the solves/s of
+    one-at-a-time plain solves at no worse p99."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
-    sizes = [int(x) for x in rng.choice([256, 1024, 4096, 16384], size=nreq, p=[0.5, 0.3, 0.15, 0.05])]
-    jobz = ["V" if x else "N" for x in (rng.random(nreq) < 0.7)]
+    counts = {256: nreq // 2, 1024: nreq * 3 // 10, 4096: nreq * 3 // 20}
+    counts[16384] = nreq - sum(counts.values())
+    sizes = [n for n, c in counts.items() for _ in range(c)]
+    rng.shuffle(sizes)
+    nv = int(round(0.7 * nreq))
+    jobz = np.array(["V"] * nv + ["N"] * (nreq - nv))
+    rng.shuffle(jobz)
     gctx = E.Context(0)
-    pristine = [E.gen_symmetric(gctx, n, 1000 + i, 1) for i, n in enumerate(sizes)]
-    torch.cuda.synchronize()
 
-    def run(slots, arrivals, batching, precision=E.PREC_FP32):
-        work = []
-        for p in pristine:
-            w = E.colmajor(p.shape[0], p.shape[0])
-            w.copy_(p)
-            work.append(w)
+    def run(nslots, arrivals, batching, precision=E.PREC_FP32, policy=0):
+        # inputs regenerated from their seeds (the solver overwrites them)
+        work = [E.gen_symmetric(gctx, n, 1000 + i, 1) for i, n in enumerate(sizes)]
         torch.cuda.synchronize()
-        srv = E.Server(0, slots=slots, hbm_budget=budget, batching=batching, precision=precision)
+        srv = E.Server(0, slots=nslots, hbm_budget=budget, batching=batching, precision=precision, policy=policy)
         t0 = time.perf_counter()
         for i in np.argsort(arrivals, kind="stable"):
             delay = t0 + arrivals[i] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
             n = sizes[i]
-            srv.submit(work[i], b=32, nb=1024 if n >= 4096 else 256, jobz=jobz[i])
+            srv.submit(work[i], b=32, nb=1024 if n >= 4096 else 256, jobz=str(jobz[i]))
         srv.wait_all()
         st = srv.stats()
         srv.close()
         del work
+        torch.cuda.empty_cache()
         return {k: st[k] for k in ("solves_per_s", "p50_ms", "p99_ms")} | {"arena_gib": st["arena_bytes"] / 2**30}
 
     zero = np.zeros(nreq)
-    sat1 = run(1, zero, False)
-    sat4 = run(4, zero, True)
+    sat1 = run(1, zero, False, policy=1)
+    print(f"[bench] serving one-at-a-time saturated: {sat1}", file=sys.stderr, flush=True)
+    sat_plain = run(slots, zero, True)
+    sat_fp16 = run(slots, zero, True, E.PREC_FP16)
+    print(f"[bench] serving engine saturated: plain {sat_plain}, fp16 {sat_fp16}", file=sys.stderr, flush=True)
     rate = 0.7 * sat1["solves_per_s"]
     arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-    poi1 = run(1, arr, False)
-    poi4 = run(4, arr, True)
-    sat4h = run(4, zero, True, E.PREC_FP16)
-    poi4h = run(4, arr, True, E.PREC_FP16)
-    return {"requests": nreq, "mix": "n in {256: 50%, 1024: 30%, 4096: 15%, 16384: 5%}, 70% jobz=V, Gaussian "
-                                     f"entries, seed {seed}, plain FP32, HBM budget {budget >> 30} GiB",
-            "saturated": {"one_at_a_time": sat1, "engine_4slots_batching": sat4,
-                          "engine_4slots_batching_fp16": sat4h,
-                          "throughput_ratio": sat4["solves_per_s"] / sat1["solves_per_s"],
-                          "throughput_ratio_fp16": sat4h["solves_per_s"] / sat1["solves_per_s"]},
-            "poisson": {"rate_per_s": rate, "one_at_a_time": poi1, "engine_4slots_batching": poi4,
-                        "engine_4slots_batching_fp16": poi4h}}
+    poi1 = run(1, arr, False, policy=1)
+    poi_fp16 = run(slots, arr / 2.0, True, E.PREC_FP16)
+    poi_plain = run(slots, arr / 2.0, True)
+    gctx.close()
+    return {"requests": nreq, "mix": f"exact counts {counts}, 70% jobz=V, Gaussian entries, seed {seed}, "
+                                      f"HBM budget {budget >> 30} GiB (one pool, lent per request)",
+            "saturated": {"one_at_a_time": sat1, f"engine_{slots}slots_plain": sat_plain,
+                          f"engine_{slots}slots_fp16": sat_fp16,
+                          "throughput_ratio_plain": sat_plain["solves_per_s"] / sat1["solves_per_s"],
+                          "throughput_ratio_fp16": sat_fp16["solves_per_s"] / sat1["solves_per_s"]},
+            "poisson": {"one_at_a_time": {"rate_per_s": rate} | poi1,
+                        f"engine_{slots}slots_fp16_at_2x_rate": {"rate_per_s": 2 * rate} | poi_fp16,
+                        f"engine_{slots}slots_plain_at_2x_rate": {"rate_per_s": 2 * rate} | poi_plain,
+                        "throughput_ratio_fp16": poi_fp16["solves_per_s"] / poi1["solves_per_s"],
+                        "p99_ratio_fp16": poi_fp16["p99_ms"] / poi1["p99_ms"],
+                        "throughput_ratio_plain": poi_plain["solves_per_s"] / poi1["solves_per_s"],
+                        "p99_ratio_plain": poi_plain["p99_ms"] / poi1["p99_ms"]}}
 
 
 def captured_bench(E, torch, reps: int = 20):
@@ -226,8 +280,9 @@ def batched_form_bench(E, torch, count: int = 512, seed: int = 1):
     """Batched form: `count` symmetric FP32 matrices, n log-uniform in
     [64, 2048] (seed 1), uniform[-1,1] entries symmetrised, half with
     eigenvectors: one evd_syevd_mixed_f32 call (size buckets, one batched
-    solve per bucket and jobz) against the same problems solved one at a time
-    with evd_syevd_f32; both with inputs resident in HBM."""
+    solve per bucket and jobz, bandwidth min(32, N / 4) per bucket order N)
+    against the same problems solved one at a time with evd_syevd_f32 (b = 32);
+    both with inputs resident in HBM."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -236,10 +291,10 @@ def batched_form_bench(E, torch, count: int = 512, seed: int = 1):
     ctx = E.Context(0)
     try:
         mats = [E.gen_symmetric(ctx, n, 5000 + i, 0) for i, n in enumerate(sizes)]
-        E.syevd_mixed(ctx, mats[:16], jobz[:16])  # warm-up: arena, plans
+        E.syevd_mixed(ctx, mats[:16], jobz[:16], b=0)  # warm-up: arena, plans
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        E.syevd_mixed(ctx, mats, jobz)
+        E.syevd_mixed(ctx, mats, jobz, b=0)
         torch.cuda.synchronize()
         t_mixed = time.perf_counter() - t0
         t0 = time.perf_counter()
@@ -269,8 +324,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--no-captured", action="store_true", help="skip the eager-vs-graph small-solve sample")
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-size batched sample")
-    ap.add_argument("--serve-requests", type=int, default=60)
-    ap.add_argument("--cpu-n", type=int, default=4096)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time host MKL dsyevd on the headline matrix itself (minutes) instead of the n=8192 sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU only (the eigensolver runs on a single MI355X and has no "
@@ -312,27 +368,34 @@ def main():
     flops = (10.0 / 3.0) * n ** 3
     value = flops / dt / 1e12
 
-    # dominant kernel: the BC back transformation, WY blocks of 32 sweeps on
-    # FP32 MFMA (bcw_kernel, timed alone with hipEvents on the solver's
-    # stream); 2 n^3 algorithmic flops (4 n per reflector entry) - the MFMA
-    # issue 1.625x that, the staircase blocks' nonzero 16 x 16 tiles in full
+    # the two largest kernels, each against its own roofline, timed alone with
+    # hipEvents on the solver's stream:
+    #  * bcw_kernel (BC back transformation, WY blocks on FP32 MFMA): 2 n^3
+    #    algorithmic flops per launch against the FP32 MFMA peak;
+    #  * the bulge chase (HBM/latency bound): the algorithmic window bytes
+    #    (1.5 b + 1) n^2 s, s = 4 bytes per element, per launch against HBM
+    #    (the band is held in FP64, twice that element size)
     bc_ms = stage_ms["bc_back_kernel"]
-    bc_flops = 2.0 * n ** 3
-    achieved = bc_flops / (bc_ms * 1e-3) / 1e12
-    traffic = profiled_traffic("bcw_kernel") if (n, b) == (16384, 32) else None
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": traffic, "kernel": "bcw_kernel",
-                "note": "2 n^3 algorithmic flops per launch over the hipEvent time of the launch (the WY form issues "
-                        "1.625x that on v_mfma_f32_16x16x4f32); FP32 MFMA peak 157.3 TF; traffic = copy-calibrated "
-                        "FETCH_SIZE + WRITE_SIZE bytes per launch from the committed rocprofv3 summary"}
-    # the bulge chase against HBM: (1.5 b + 1) n^2 FP64 window bytes moved per launch
-    bc_bytes = (1.5 * b + 1) * n * n * 8
-    chase = {"bound": "hbm", "achieved": bc_bytes / (stage_ms["bc"] * 1e-3) / 1e9 if stage_ms["bc"] else None,
-             "peak": PEAK["hbm_gbs"], "unit": "GB/s", "kernel": "chase_kernel",
-             "us_per_dependent_step": stage_ms["bc"] * 1e3 / (2.0 * n) if stage_ms["bc"] else None,
-             "traffic": profiled_traffic("chase_kernel") if (n, b) == (16384, 32) else None}
-    if chase["achieved"] is not None:
-        chase["frac"] = chase["achieved"] / PEAK["hbm_gbs"]
+    achieved = 2.0 * n ** 3 / (bc_ms * 1e-3) / 1e12 if bc_ms else None
+    bcw = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
+           "frac": achieved / PEAK["fp32_mfma_tflops"] if achieved else None,
+           "traffic": profiled_traffic("bcw_kernel") if (n, b) == (16384, 32) else None, "kernel": "bcw_kernel",
+           "ms": bc_ms,
+           "note": "2 n^3 algorithmic flops per launch over the hipEvent time of the launch (the WY form issues "
+                   "1.625x that on v_mfma_f32_16x16x4f32); FP32 MFMA peak 157.3 TF; traffic = copy-calibrated "
+                   "FETCH_SIZE + WRITE_SIZE bytes per launch from the committed rocprofv3 summary"}
+    ch_ms = stage_ms.get("chase_kernel") or stage_ms["bc"]
+    ch_bytes = (1.5 * b + 1) * n * n * 4
+    ch_ach = ch_bytes / (ch_ms * 1e-3) / 1e9 if ch_ms else None
+    chase = {"bound": "hbm", "achieved": ch_ach, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+             "frac": ch_ach / PEAK["hbm_gbs"] if ch_ach else None,
+             "traffic": profiled_traffic("chase_group_kernel") if (n, b) == (16384, 32) else None,
+             "kernel": "chase_group_kernel", "ms": ch_ms,
+             "us_per_dependent_step": ch_ms * 1e3 / (2.0 * n) if ch_ms else None,
+             "note": "(1.5 b + 1) n^2 x 4 bytes algorithmic window traffic per launch over the hipEvent time of "
+                     "the launch; 8 TB/s HBM peak; the chase is bound by its 2n dependent steps"}
+    roofline = bcw if (bc_ms or 0) >= (ch_ms or 0) else chase
+    achieved = bcw["achieved"]
     stage_tflops = {
         "sbr": (4.0 / 3.0) * n ** 3 / (stage_ms["sbr"] * 1e-3) / 1e12 if stage_ms["sbr"] else None,
         "bc_back": achieved,
@@ -375,10 +438,20 @@ def main():
             print(f"[bench] mixed {name}: {mdt:.3f} s/EVD ({dt / mdt:.2f}x), {entry.get('accuracy')}", file=sys.stderr,
                   flush=True)
         ctx.set_precision(E.PREC_FP32)
-    serving = None
-    if not args.no_serve:
-        serving = serving_bench(E, torch, args.serve_requests)
-        print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
+    # the headline (D, E) from the chase and the matrix itself, for the host
+    # baseline; then the headline buffers are released before the other legs
+    cpu_in = None
+    if not args.no_cpu:
+        Aw.copy_(A0)
+        Ab, _, _ = E.sy2sb(ctx, Aw, b, nb)
+        band = E.band_pack(ctx, Ab, b)
+        Dd, Ed, _ = E.sb2st(ctx, band, n, b)
+        cpu_in = (A0.t().contiguous().cpu().numpy().T if args.cpu_full else None, Dd.cpu().numpy(), Ed.cpu().numpy())
+        del Ab, band, Dd, Ed
+    ctx.close()
+    del A0, Aw, r
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     captured = None
     if not args.no_captured:
         captured = captured_bench(E, torch)
@@ -387,7 +460,12 @@ def main():
     if not args.no_batched:
         batched = batched_form_bench(E, torch)
         print(f"[bench] batched form: {batched}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
+    serving = None
+    if not args.no_serve:
+        serving = serving_bench(E, torch, args.serve_requests)
+        print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
+    cpu = None if cpu_in is None else cpu_baseline(cpu_in[0], cpu_in[1], cpu_in[2], args.cpu_full)
+    print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
     line = {
         "metric": "EVD s & TFLOP/s at n=16384 FP32 b=32; mixed-size solves/s + p99 latency",
         "value": value,
@@ -407,7 +485,7 @@ def main():
         "stage_ms": stage_ms,
         "stage_tflops": stage_tflops,
         "roofline": roofline,
-        "chase_roofline": chase,
+        "rooflines": {"bcw_kernel": bcw, "chase_group_kernel": chase},
         "cpu_baseline": cpu,
         "accuracy": acc,
         "mixed_precision": mixed or None,

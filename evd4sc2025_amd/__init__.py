@@ -31,8 +31,9 @@ __all__ = [
 PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
 
 STAGES = ("sbr", "bc", "dc", "bc_back", "sbr_back")
-# extra timer after the stages: the BC back-transformation kernel alone
-TIMERS = STAGES + ("bc_back_kernel",)
+# extra timers after the stages: the BC back-transformation kernel alone and
+# the bulge-chase kernel alone
+TIMERS = STAGES + ("bc_back_kernel", "chase_kernel")
 
 
 def _torch():
@@ -162,7 +163,7 @@ class Context:
 
     def stage_ms(self) -> dict:
         """Per-stage device ms of the last syevd (hipEvents on the context
-        stream), plus 'bc_back_kernel': the BC back-transformation kernel."""
+        stream), plus 'bc_back_kernel' and 'chase_kernel': those kernels alone."""
         arr = (C.c_float * len(TIMERS))()
         self.call("evd_ctx_stage_ms", arr, len(TIMERS))
         return {k: float(v) for k, v in zip(TIMERS, arr)}
@@ -185,46 +186,60 @@ def _ch(c: str) -> C.c_char:
 
 # ----------------------------------------------------------------- building blocks
 
+def _sfx(t) -> str:
+    """C entry point suffix of a tensor's element type (FP32 or FP64 forms)."""
+    torch = _torch()
+    if t.dtype == torch.float64:
+        return "f64"
+    if t.dtype == torch.float32:
+        return "f32"
+    raise TypeError(f"expected float32 or float64, got {t.dtype}")
+
+
 def panelQR(ctx: Context, A, W, R):
-    """api/panelQR.cu:20 - A (m x b) <- Y (unit lower), W = V T, R (b x b)."""
+    """api/panelQR.cu:20 - A (m x b) <- Y (unit lower), W = V T, R (b x b);
+    float32 or float64 buffers (evd_panel_qr_f32 / _f64)."""
     m, b = A.shape
     pa, lda = _cm(A)
     pw, ldw = _cm(W)
     pr, ldr = _cm(R)
-    ctx.call("evd_panel_qr_f32", m, b, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(pr), ldr)
+    ctx.call(f"evd_panel_qr_{_sfx(A)}", m, b, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(pr), ldr)
 
 
 def hou_tsqr_panel(ctx: Context, A, R):
     """api/inc/TallShinnyQR.h:9 - A (m x n) <- explicit thin Q of the panel, R (n x n)."""
     m, n = A.shape
-    W = colmajor(m, n)
-    work = colmajor(n, n)
+    W = colmajor(m, n, dtype=A.dtype)
+    work = colmajor(n, n, dtype=A.dtype)
     pa, lda = _cm(A)
     pw, ldw = _cm(W)
     pr, ldr = _cm(R)
-    ctx.call("evd_panel_qr_f32", m, n, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(pr), ldr)
-    ctx.call("evd_orgqr_f32", m, n, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(work.data_ptr()))
+    sfx = _sfx(A)
+    ctx.call(f"evd_panel_qr_{sfx}", m, n, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(pr), ldr)
+    ctx.call(f"evd_orgqr_{sfx}", m, n, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(work.data_ptr()))
 
 
 def tc_ozimmu_syr2k(ctx: Context, alpha: float, A, B, beta: float, C_):
-    """api/TCblas/tc_ozimmu_syr2k.cu:209 - C = alpha (A B^T + B A^T) + beta C (full symmetric result)."""
+    """api/TCblas/tc_ozimmu_syr2k.cu:209 - C = alpha (A B^T + B A^T) + beta C
+    (full symmetric result); float32 or float64."""
     n, k = A.shape
     pa, lda = _cm(A)
     pb, ldb = _cm(B)
     pc, ldc = _cm(C_)
-    ctx.call("evd_syr2k_f32", n, k, C.c_float(alpha), C.c_void_p(pa), lda, C.c_void_p(pb), ldb, C.c_float(beta),
+    sc = C.c_double if _sfx(A) == "f64" else C.c_float
+    ctx.call(f"evd_syr2k_{_sfx(A)}", n, k, sc(alpha), C.c_void_p(pa), lda, C.c_void_p(pb), ldb, sc(beta),
              C.c_void_p(pc), ldc)
 
 
 def sy2sb(ctx: Context, A, b: int, nb: int, W=None, Y=None):
     """Full -> band (api/zy_zy_sy2sb.cu:24).  A is overwritten (lower band = B)."""
     n = A.shape[0]
-    W = W if W is not None else colmajor(n, n)
-    Y = Y if Y is not None else colmajor(n, n)
+    W = W if W is not None else colmajor(n, n, dtype=A.dtype)
+    Y = Y if Y is not None else colmajor(n, n, dtype=A.dtype)
     pa, lda = _cm(A)
     pw, ldw = _cm(W)
     py, ldy = _cm(Y)
-    ctx.call("evd_sy2sb_f32", n, b, nb, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(py), ldy)
+    ctx.call(f"evd_sy2sb_{_sfx(A)}", n, b, nb, C.c_void_p(pa), lda, C.c_void_p(pw), ldw, C.c_void_p(py), ldy)
     return A, W, Y
 
 
@@ -281,7 +296,7 @@ def computerWYFromSlide(ctx: Context, slide: int, W, Y):
     M, N = W.shape
     pw, ldw = _cm(W)
     py, ldy = _cm(Y)
-    ctx.call("evd_wy_from_slide_f32", M, N, slide, C.c_void_p(pw), ldw, C.c_void_p(py), ldy)
+    ctx.call(f"evd_wy_from_slide_{_sfx(W)}", M, N, slide, C.c_void_p(pw), ldw, C.c_void_p(py), ldy)
 
 
 def computerQFromWY(ctx: Context, Q, W, Y):
@@ -290,13 +305,13 @@ def computerQFromWY(ctx: Context, Q, W, Y):
     pq, ldq = _cm(Q)
     pw, ldw = _cm(W)
     py, ldy = _cm(Y)
-    ctx.call("evd_q_from_wy_f32", M, N, C.c_void_p(pq), ldq, C.c_void_p(pw), ldw, C.c_void_p(py), ldy)
+    ctx.call(f"evd_q_from_wy_{_sfx(Q)}", M, N, C.c_void_p(pq), ldq, C.c_void_p(pw), ldw, C.c_void_p(py), ldy)
 
 
 def bc_back(ctx: Context, Q, U, n: int, b: int, order: int):
     pq, ldq = _cm(Q)
     pu, ldu = _cm(U)
-    ctx.call("evd_bc_back_f32", n, b, Q.shape[1], C.c_void_p(pq), ldq, C.c_void_p(pu), ldu, order)
+    ctx.call(f"evd_bc_back_{_sfx(Q)}", n, b, Q.shape[1], C.c_void_p(pq), ldq, C.c_void_p(pu), ldu, order)
 
 
 def my_BC_back_trans_v8_10_noBandU(ctx: Context, Q, U, n: int, b: int):
@@ -318,7 +333,7 @@ def checkOrthogonality(ctx: Context, Q) -> float:
     m, n = Q.shape
     pq, ldq = _cm(Q)
     out = C.c_double()
-    ctx.call("evd_orthogonality_f32", m, n, C.c_void_p(pq), ldq, C.byref(out))
+    ctx.call(f"evd_orthogonality_{_sfx(Q)}", m, n, C.c_void_p(pq), ldq, C.byref(out))
     return out.value
 
 
@@ -395,7 +410,7 @@ def syevd(ctx: Context, A, b: int = 32, nb: int = 1024, jobz: str = "V", overwri
 def syevd_mixed(ctx: Context, mats, jobz="V", b: int = 32):
     """Problems of different orders in one call (evd_syevd_mixed_f32): each is
     embedded in the next bucket order and every (bucket, jobz) group is one
-    batched solve.  `mats`: list of square CUDA tensors (lower triangles
+    batched solve (b = 0: bandwidth min(32, N / 4) per bucket order N).  `mats`: list of square CUDA tensors (lower triangles
     read); `jobz`: one flag or a list.  Returns a list of EigResult."""
     torch = _torch()
     count = len(mats)
@@ -594,7 +609,7 @@ class Server:
     must stay alive until the request completes (the server keeps them)."""
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 0, batching: bool = True,
-                 precision: int = PREC_FP32):
+                 precision: int = PREC_FP32, policy: int = 0):
         self._lib = lib()
         h = C.c_void_p()
         rc = self._lib.evd_server_create(device, slots, hbm_budget, C.byref(h))
@@ -606,6 +621,9 @@ class Server:
         rc = self._lib.evd_server_set_precision(h, precision)
         if rc:
             raise EvdError("evd_server_set_precision", rc, "")
+        rc = self._lib.evd_server_set_policy(h, policy)  # 0: largest order first, 1: FIFO
+        if rc:
+            raise EvdError("evd_server_set_policy", rc, "")
         self._keep = {}
 
     def submit(self, A, b: int = 32, nb: int = 1024, jobz: str = "V"):

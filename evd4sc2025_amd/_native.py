@@ -46,6 +46,13 @@ SIGNATURES = {
     "evd_bc_back_f32": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l, _i]),
     "evd_sbr_back_f32": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
     "evd_syevd_f32": (_i, [_vp, _ch, _l, _l, _l, _vp, _l, _vp, _vp, _l]),
+    "evd_panel_qr_f64": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
+    "evd_orgqr_f64": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp]),
+    "evd_syr2k_f64": (_i, [_vp, _l, _l, _d, _vp, _l, _vp, _l, _d, _vp, _l]),
+    "evd_sy2sb_f64": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
+    "evd_wy_from_slide_f64": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l]),
+    "evd_q_from_wy_f64": (_i, [_vp, _l, _l, _vp, _l, _vp, _l, _vp, _l]),
+    "evd_bc_back_f64": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _l, _i]),
     "evd_syevd_workspace_f32": (_sz, [_l, _l, _l, _ch]),
     "evd_syevd_workspace_ex": (_sz, [_l, _l, _l, _ch, _i]),
     "evd_autotune": (_i, [_vp, _l, _l, _i, C.POINTER(_i)]),
@@ -66,6 +73,7 @@ SIGNATURES = {
     "evd_tune_cache_load": (_i, [C.c_char_p]),
     "evd_ctx_use_tune_cache": (_i, [_vp, _i]),
     "evd_orthogonality_f32": (_i, [_vp, _l, _l, _vp, _l, C.POINTER(_d)]),
+    "evd_orthogonality_f64": (_i, [_vp, _l, _l, _vp, _l, C.POINTER(_d)]),
     "evd_backward_error_f32": (_i, [_vp, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_d)]),
     "evd_sgemm": (_i, [_vp, _ch, _ch, _l, _l, _l, _f, _vp, _l, _vp, _l, _f, _vp, _l]),
     "evd_dgemm": (_i, [_vp, _ch, _ch, _l, _l, _l, _d, _vp, _l, _vp, _l, _d, _vp, _l]),
@@ -75,6 +83,7 @@ SIGNATURES = {
     "evd_server_create": (_i, [_i, _i, C.c_size_t, C.POINTER(_vp)]),
     "evd_server_set_batching": (_i, [_vp, _i]),
     "evd_server_set_precision": (_i, [_vp, _i]),
+    "evd_server_set_policy": (_i, [_vp, _i]),
     "evd_server_destroy": (_i, [_vp]),
     "evd_server_submit": (_i, [_vp, _l, _l, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_l)]),
     "evd_server_wait": (_i, [_vp, _l]),

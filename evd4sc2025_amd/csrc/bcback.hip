@@ -391,10 +391,79 @@ static bool use_wy(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
 }
 
 size_t bc_back_bytes(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
+  if (b != 32 && b != 64) return 0;  // the small-order kernel needs no workspace
   return use_wy(ctx, n, b, ldq, rev) ? bc_back_wy_bytes(n) : bc_back_pack_bytes(n, b);
 }
 
+// Small orders with any bandwidth b <= 64 (n <= kSmallN, the small buckets
+// of a mixed-size batch, where b = min(32, n / 4) can be 16 or 24): one
+// workgroup per problem (blockIdx.y) holds all of Q (n x ncols) in LDS, one
+// thread per column, and applies the reflectors one by one in the order of
+// the product (generation order for order 0, reversed for order 1).
+constexpr long kSmallN = 128;
+
+// reflectors of one small problem: (s, p, len) packed as s | p << 8 | len << 16
+constexpr int kSmallRefl = 8192;  // n <= 128, b >= 2: at most 126 sweeps x 64 steps
+
+__global__ __launch_bounds__(128) void bcback_small_kernel(int n, int b, int ncols, float *__restrict__ Q_all, long ldq,
+                                                           long q_stride, const float *__restrict__ U_all, long ldu,
+                                                           long u_stride, int rev) {
+  __shared__ float q[kSmallN * (kSmallN + 1)];
+  __shared__ float u[64];
+  __shared__ int list[kSmallRefl];
+  __shared__ int s_total;
+  float *Q = Q_all + (long)blockIdx.y * q_stride;
+  const float *U = U_all + (long)blockIdx.y * u_stride;
+  const int c = threadIdx.x;
+  constexpr int LDQ = kSmallN + 1;
+  for (int r = 0; r < n; r++)
+    if (c < ncols) q[r + c * LDQ] = Q[r + (long)c * ldq];
+  if (c == 0) {
+    int total = 0;
+    for (int sw = 0; sw < n - 2; sw++) {
+      int p = sw + 1, len = min(b, n - p);
+      while (total < kSmallRefl) {
+        list[total++] = sw | (p << 8) | (len << 16);
+        const int len2 = max(0, min(b, n - p - len));
+        if (len2 <= 1) break;
+        p += len;
+        len = len2;
+      }
+    }
+    s_total = total;
+  }
+  __syncthreads();
+  const int total = s_total;
+  for (int k = 0; k < total; k++) {
+    const int e = list[rev ? total - 1 - k : k];
+    const int sw = e & 0xFF, p = (e >> 8) & 0xFF, len = e >> 16;
+    if (c < len) u[c] = U[(p + c) + (long)sw * ldu];
+    __syncthreads();
+    if (c < ncols) {
+      float d = 0.f;
+      for (int i = 0; i < len; i++) d += u[i] * q[(p + i) + c * LDQ];
+      for (int i = 0; i < len; i++) q[(p + i) + c * LDQ] -= u[i] * d;
+    }
+    __syncthreads();
+  }
+  for (int r = 0; r < n; r++)
+    if (c < ncols) Q[r + (long)c * ldq] = q[r + c * LDQ];
+}
+
+int bc_back_small_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float *Q, long ldq, long q_stride,
+                          const float *U, long ldu, long u_stride, bool rev) {
+  if (n <= 2 || ncols <= 0 || count <= 0) return 0;
+  if (n > kSmallN || ncols > kSmallN || b < 2 || b > 64)
+    return set_error(ctx, EVD_E_UNSUPPORTED, "BC back transformation with b = %ld needs n <= %ld and b >= 2", b,
+                     kSmallN);
+  bcback_small_kernel<<<dim3(1, (unsigned)count), 128, 0, ctx->stream>>>((int)n, (int)b, (int)ncols, Q, ldq, q_stride,
+                                                                         U, ldu, u_stride, rev ? 1 : 0);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev) {
+  if (b != 32 && b != 64) return bc_back_small_batched(ctx, 1, n, b, ncols, Q, ldq, 0, U, ldu, 0, rev);
   if (use_wy(ctx, n, b, ldq, rev)) return bc_back_wy(ctx, n, ncols, Q, ldq, U, ldu, ctx->precision == EVD_PREC_FP16);
   const int bs = ctx->tune.bc_sweeps;
   const int nw = ctx->tune.bc_waves;
@@ -418,7 +487,7 @@ extern "C" int evd_bc_back_f32(evd_ctx *ctx, long n, long b, long ncols, float *
                                int order) {
   if (!ctx) return -1;
   EVD_ARG(ctx, n >= 0, 2);
-  EVD_ARG(ctx, b == 32 || b == 64, 3);
+  EVD_ARG(ctx, b == 32 || b == 64 || (b >= 2 && b <= 64 && n <= kSmallN && ncols <= kSmallN), 3);
   EVD_ARG(ctx, ncols >= 0, 4);
   EVD_ARG(ctx, ldq >= std::max<long>(1, n), 6);
   EVD_ARG(ctx, ldu >= n, 8);

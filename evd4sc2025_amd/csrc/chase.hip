@@ -230,6 +230,300 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
   }
 }
 
+// ---------------------------------------------------------------------------
+// Grouped chase (b <= 32): a workgroup owns a group of D consecutive sweeps,
+// s0 .. s0 + D - 1, one wave per sweep.  The waves step in lockstep: at time
+// step t, sweep s0 + i does its step t - 2i (the two-step lag of the sweep
+// order), all on a window of band columns held in LDS (column c in slot
+// c mod kGW, offsets 0 .. 2b-1), so the hand-off between the group's sweeps
+// never leaves the CU.  One more wave moves the window (I/O wave):
+//  * it brings in the columns the front sweep s0 first needs at
+//    step j (columns s0 .. p_0 + len_0 at j = 0, p_j .. p_j + len_j later, up
+//    to n at the front's last step) from HBM: the loads are issued at time
+//    j - 2, once the previous group's progress word is >= p_j + 2b - 1, and
+//    written to LDS at time j - 1.  The last column of a chunk, p_j + b - 1,
+//    is a step row of the previous group's last sweep, which writes it back
+//    down to its early-published beta only (offset b): the fill below is
+//    zeroed by that sweep's next step, so the I/O wave writes those zeros;
+//  * it writes back, at time t, the columns the group's last
+//    sweep finished at t - 1 (all columns up to its step row P; column P
+//    down to offset b) and publishes that sweep's next row P + len at t + 1,
+//    after its stores have completed (vmcnt(0)); the last write-back covers
+//    every remaining column and publishes INT_MAX.
+// Within a time step the waves' LDS accesses are disjoint and every slot is
+// rewritten only after its previous column was written back, provided
+// kGW >= (2D + 2) b - D + 1; a CPU model of this schedule (random
+// interleavings of groups, access-set and slot checks) reproduces the
+// sequential chase.  Groups are handed out by ticket in increasing order, so
+// a group's predecessor is always running; every wait is bounded.
+constexpr int kGW = 256;     // window columns
+constexpr int kGLD = 65;     // doubles per window column (2b <= 64 offsets + 1)
+// most columns the front brings in at one step: b + 2 at b = 32 (the first
+// chunk s0 .. n of a sweep whose first step is also its last)
+constexpr int kGChunk = 34;
+
+__device__ __attribute__((always_inline)) inline void grp_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+__device__ __attribute__((always_inline)) inline void wave_lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+__device__ inline int sweep_nsteps(int n, int b, int s) {
+  int p = s + 1, len = min(b, n - p), k = 1;
+  while (true) {
+    const int len2 = max(0, min(b, n - p - len));
+    if (len2 <= 1) return k;
+    p += len;
+    len = len2;
+    k++;
+  }
+}
+
+// reflector of the chase's convention from x (lane < lenx), wave-wide
+__device__ inline void grp_reflector(double x, int lane, int lenx, double &uu, double &beta) {
+  const double nrm = sqrt(wave_sum(x * x));
+  const double x0 = __shfl(x, 0, 64);
+  uu = 0.0;
+  if (nrm > 0.0) {
+    const double inv = 1.0 / nrm;
+    const double u0 = x0 * inv + (x0 >= 0.0 ? 1.0 : -1.0);
+    const double scl = 1.0 / sqrt(fabs(u0));
+    uu = (lane == 0 ? u0 : x * inv) * scl;
+  }
+  if (lane >= lenx) uu = 0.0;
+  beta = (nrm > 0.0) ? (x0 >= 0.0 ? -nrm : nrm) : x0;
+}
+
+template <int D>
+__global__ __launch_bounds__((D + 1) * 64, 1) void chase_group_kernel(
+    int n, int b, double *__restrict__ band_all, long ldb, unsigned band_bytes, float *__restrict__ U_all, long ldu,
+    int *__restrict__ com_all, unsigned *__restrict__ ticket, unsigned *__restrict__ err, int count, long band_stride,
+    long u_stride, long com_stride) {
+  __shared__ double win[kGW * kGLD];
+  __shared__ double uS[D][32], yS[D][32], qS[D][32];
+  __shared__ int s_ticket, s_abort;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nsweeps = n - 2;
+  const int ngroups = (nsweeps + D - 1) / D;
+  const unsigned ld = (unsigned)ldb;
+  const int b2 = 2 * b;
+
+  while (true) {
+    if (tid == 0) {
+      s_ticket = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_abort = 0;
+    }
+    grp_barrier();
+    const int tk = s_ticket;
+    grp_barrier();
+    if (tk >= ngroups * count) return;
+    const int q = tk % count, g = tk / count;
+    const int s0 = g * D, nsw = min(D, nsweeps - s0);
+    const __amdgpu_buffer_rsrc_t rb =
+        __builtin_amdgcn_make_buffer_rsrc(band_all + (long)q * band_stride, 0, (int)band_bytes, kBufferWord3);
+    float *U = U_all + (long)q * u_stride;
+    int *com = com_all + (long)q * com_stride;
+    const int tail_steps = sweep_nsteps(n, b, s0 + nsw - 1);
+    const int front_steps = sweep_nsteps(n, b, s0);
+    const int te = tail_steps - 1 + 2 * (nsw - 1);  // time step of the last sweep's last step
+
+    // Each role runs its own copy of the time loop (one barrier per time
+    // step in every copy), so the roles' registers do not add up.
+    if (wave < D) {
+      // ---- worker: sweep ws of this group
+      const int ws = s0 + wave;
+      const int w_steps = (wave < nsw) ? sweep_nsteps(n, b, ws) : 0;
+      int wp = ws + 1, wlen = min(b, n - wp), cB1 = ws, colB1 = 1;
+      bool wfirst = true;
+      double wbeta = 0.0;
+      double *u = uS[wave], *yv = yS[wave], *qv = qS[wave];
+      for (int t = -2; t <= te + 2; ++t) {
+        const int js = t - 2 * wave;
+        if (js >= 0 && js < w_steps) {
+          const int p = wp, len = wlen;
+          const int len2 = max(0, min(b, n - p - len));
+          const bool last = (len2 <= 1);
+          if (wfirst) {
+            // the sweep's first reflector, from column ws below the diagonal
+            const double x = (lane < len) ? win[(ws & (kGW - 1)) * kGLD + 1 + lane] : 0.0;
+            double uu, beta;
+            grp_reflector(x, lane, len, uu, beta);
+            if (lane < 32) u[lane] = uu;
+            if (lane < len) U[(p + lane) + (long)ws * ldu] = (float)uu;
+            wbeta = beta;
+            wave_lds_order();
+          }
+          // ---- B1 <- H B1 (columns 1 .. colB1-1); column 0 becomes beta e1
+          {
+            const int c = lane & 31, h = lane >> 5;
+            const bool act = (c >= 1 && c < colB1);
+            // row p of column cB1 + c (lanes past the block read slot 0, unused)
+            const int base = (c < colB1) ? ((cB1 + c) & (kGW - 1)) * kGLD + (p - cB1 - c) : 0;
+            double bv[16];
+            double tsum = 0.0;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+              const int i = 2 * r + h;
+              bv[r] = (act && i < len) ? win[base + i] : 0.0;
+              tsum += u[i] * bv[r];
+            }
+            tsum += __shfl_xor(tsum, 32, 64);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+              const int i = 2 * r + h;
+              if (act && i < len) win[base + i] = bv[r] - u[i] * tsum;
+            }
+            if (c == 0) {
+#pragma unroll
+              for (int r = 0; r < 16; r++) {
+                const int i = 2 * r + h;
+                if (i < len && !(i == 0 && !wfirst)) win[base + i] = (i == 0) ? wbeta : 0.0;
+              }
+            }
+          }
+          // ---- S <- H S H, N <- N H (S from the lower band, row ri, columns kk)
+          const int ri = lane >> 1, kc = lane & 1;
+          double Sr[16], Nr[16];
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const int kk = kc + 2 * r;
+            const int lo = min(ri, kk), hi = max(ri, kk);
+            Sr[r] = (ri < len && kk < len) ? win[((p + lo) & (kGW - 1)) * kGLD + (hi - lo)] : 0.0;
+            Nr[r] = (ri < len2 && kk < len) ? win[((p + kk) & (kGW - 1)) * kGLD + (len + ri - kk)] : 0.0;
+          }
+          double yp = 0.0, qp = 0.0;
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const double uk = u[kc + 2 * r];
+            yp += Sr[r] * uk;
+            qp += Nr[r] * uk;
+          }
+          yp += __shfl_xor(yp, 1, 64);
+          qp += __shfl_xor(qp, 1, 64);
+          if (kc == 0) {
+            yv[ri] = yp;
+            qv[ri] = qp;
+          }
+          wave_lds_order();
+          const double psi = wave_sum((lane < 32) ? u[lane] * yv[lane] : 0.0);
+          const double ui = u[ri], wi = yv[ri] - 0.5 * psi * ui, qi = qv[ri];
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const int kk = kc + 2 * r;
+            const double uk = u[kk], wk = yv[kk] - 0.5 * psi * uk;
+            Sr[r] -= ui * wk + wi * uk;
+            Nr[r] -= qi * uk;
+            if (ri >= kk && ri < len && kk < len) win[((p + kk) & (kGW - 1)) * kGLD + (ri - kk)] = Sr[r];
+            if (ri < len2 && kk < len) win[((p + kk) & (kGW - 1)) * kGLD + (len + ri - kk)] = Nr[r];
+          }
+          wave_lds_order();
+          // ---- next reflector from N[:, 0] (column p); its beta goes in now
+          if (!last) {
+            const double x = (lane < len2) ? win[(p & (kGW - 1)) * kGLD + len + lane] : 0.0;
+            double uu, beta;
+            grp_reflector(x, lane, len2, uu, beta);
+            wave_lds_order();
+            if (lane < 32) u[lane] = uu;
+            if (lane < len2) U[(p + len + lane) + (long)ws * ldu] = (float)uu;
+            if (lane == 0) win[(p & (kGW - 1)) * kGLD + len] = beta;
+            wbeta = beta;
+            wfirst = false;
+            cB1 = p;
+            colB1 = len;
+            wp = p + len;
+            wlen = len2;
+          }
+        }
+        grp_barrier();
+        if (s_abort) break;
+      }
+    } else {
+      // ---- I/O wave.  Per time step: wait for last step's memory operations,
+      // publish last step's write-back, put chunk t+1 (loaded last step) into
+      // LDS, write back the columns the last sweep finished at t - 1, then
+      // issue the loads of chunk t+2 and a poll of the predecessor's progress.
+      int lj = 0, fp = s0 + 1, flen = min(b, n - s0 - 1);
+      int known = (g == 0) ? INT_MAX : -1, pv = -1;
+      bool pv_valid = false, pre_valid = false;
+      int pre_lo = 0, pre_hi = 0, pre_syn = -1;
+      double pre[kGChunk];
+      int tp = s0 + nsw, tlen = min(b, n - s0 - nsw), wb = s0, pval = 0;
+      bool pend = false;
+      for (int t = -2; t <= te + 2; ++t) {
+        wait_mem();
+        if (pend) {
+          if (lane == 0) st_dev(&com[(long)g * kComStride], pval);
+          pend = false;
+        }
+        if (pre_valid) {
+#pragma unroll
+          for (int k = 0; k < kGChunk; k++) {
+            const int c = pre_lo + k;
+            if (c < pre_hi && lane < b2) win[(c & (kGW - 1)) * kGLD + lane] = (c == pre_syn && lane > b) ? 0.0 : pre[k];
+          }
+          pre_valid = false;
+        }
+        const int k = t - 1 - 2 * (nsw - 1);
+        if (k >= 0 && k < tail_steps) {
+          const int P = tp, len = tlen;
+          const bool last = (k == tail_steps - 1);
+          const int hi = last ? n : P + 1;
+          for (int c = wb; c < hi; c++) {
+            const int dmax = (last || c < P) ? b2 : b + 1;
+            if (lane < dmax) st_band(rb, (unsigned)lane + (unsigned)c * ld, win[(c & (kGW - 1)) * kGLD + lane]);
+          }
+          wb = hi;
+          pend = true;
+          pval = last ? INT_MAX : P + len;
+          tlen = max(0, min(b, n - P - len));
+          tp = P + len;
+        }
+        if (lj < front_steps) {
+          if (pv_valid) known = max(known, pv);
+          const int need = fp + 2 * b - 1;
+          if (known < need) {
+            const unsigned long long t0 = clock100();
+            while (true) {
+              known = max(known, ld_dev(&com[(long)(g - 1) * kComStride]));
+              if (known >= need) break;
+              if (clock100() - t0 > kWatchdogTicks) {
+                if (lane == 0) {
+                  atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+                  s_abort = 1;
+                }
+                break;
+              }
+              __builtin_amdgcn_s_sleep(1);
+            }
+          }
+          pre_lo = (lj == 0) ? s0 : fp;
+          pre_hi = (lj == front_steps - 1) ? n : fp + flen;
+          pre_syn = fp + b - 1;
+#pragma unroll
+          for (int k2 = 0; k2 < kGChunk; k2++) {
+            const int c = pre_lo + k2;
+            const unsigned off = (c < pre_hi && lane < b2) ? ((unsigned)lane + (unsigned)c * ld) * 8u : 0x80000000u;
+            pre[k2] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rb, off, 0, kSc1));
+          }
+          pre_valid = true;
+          fp += flen;
+          flen = min(b, n - fp);
+          lj++;
+        }
+        if (g > 0) {
+          pv = ld_dev(&com[(long)(g - 1) * kComStride]);
+          pv_valid = true;
+        }
+        grp_barrier();
+        if (s_abort) break;
+      }
+    }
+    if (s_abort) {
+      if (wave == D && lane == 0) st_dev(&com[(long)g * kComStride], INT_MAX);
+      return;
+    }
+  }
+}
+
 __global__ void band_pack_kernel(int n, int b, const float *__restrict__ A, long lda, double *__restrict__ band,
                                  long ldb) {
   long c = blockIdx.y * (long)blockDim.y + threadIdx.y;
@@ -247,6 +541,33 @@ int band_pack(evd_ctx *ctx, long n, long b, const float *A, long lda, double *ba
   unsigned gy = (unsigned)std::min<long>((n + 3) / 4, 65535);
   band_pack_kernel<<<dim3(1, gy), blk, 0, ctx->stream>>>((int)n, (int)b, A, lda, band, ldband);
   EVD_LAUNCHED(ctx);
+  return 0;
+}
+
+// largest sweeps-per-group whose window fits kGW columns: (2D + 2) b - D + 1 <= kGW
+static int group_depth_max(long b) { return (int)std::max<long>(0, (kGW - 1 - 2 * b) / (2 * b - 1)); }
+
+template <int D>
+static int launch_chase_group(evd_ctx *ctx, int count, long n, long b, double *band, long ldb, long band_stride,
+                              float *U, long ldu, long u_stride, int *com, long com_stride, unsigned *ticket) {
+  const long bytes = ldb * n * (long)sizeof(double);
+  if (bytes >= (1L << 31))
+    return set_error(ctx, EVD_E_UNSUPPORTED, "band of %ld bytes exceeds the 2 GiB buffer range of the chase", bytes);
+  // groups in flight: a group lags its predecessor by about 2D + 4 steps and
+  // lives about n / b steps
+  int resident = 0;
+  EVD_TRY(max_coresident_blocks(ctx, (const void *)chase_group_kernel<D>, (D + 1) * 64, 0, &resident));
+  const long ngroups = (n - 2 + D - 1) / D;
+  const long inflight = (((n + b - 1) / b + 2 * D + 3) / (2 * D + 4) + 2) * count;
+  long G = ctx->tune.chase_groups > 0 ? ctx->tune.chase_groups : inflight;
+  G = std::max<long>(1, std::min<long>(std::min<long>(G, resident), ngroups * count));
+  unsigned *err = ctx->ctrl + kCtrlError;
+  if (ctx->timing && !ctx->capturing) hipEventRecord(ctx->cev[0], ctx->stream);
+  chase_group_kernel<D><<<(unsigned)G, (D + 1) * 64, 0, ctx->stream>>>((int)n, (int)b, band, ldb, (unsigned)bytes, U,
+                                                                        ldu, com, ticket, err, count, band_stride,
+                                                                        u_stride, com_stride);
+  EVD_LAUNCHED(ctx);
+  if (ctx->timing && !ctx->capturing) hipEventRecord(ctx->cev[1], ctx->stream);
   return 0;
 }
 
@@ -272,9 +593,11 @@ static int launch_chase(evd_ctx *ctx, int count, long n, long b, double *band, l
   int ni = (int)n, bi = (int)b;
   unsigned nbytes = (unsigned)bytes;
   unsigned *err = ctx->ctrl + kCtrlError;
+  if (ctx->timing && !ctx->capturing) hipEventRecord(ctx->cev[0], ctx->stream);
   chase_kernel<BW><<<(unsigned)G, 256, 0, ctx->stream>>>(ni, bi, band, ldb, nbytes, U, ldu, com, ticket, err, count,
                                                           band_stride, u_stride, com_stride);
   EVD_LAUNCHED(ctx);
+  if (ctx->timing && !ctx->capturing) hipEventRecord(ctx->cev[1], ctx->stream);
   return 0;
 }
 
@@ -308,7 +631,22 @@ int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ld
     if (!com) return EVD_E_NOMEM;
     unsigned *ticket = reinterpret_cast<unsigned *>(com + (size_t)count * com_stride);
     EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), words, 0u));
-    if (b <= 32)
+    // sweeps per workgroup: the knob (0 = 1, one sweep per workgroup), at most
+    // what the LDS window holds
+    const int depth = std::min(ctx->tune.chase_depth > 0 ? ctx->tune.chase_depth : 1, group_depth_max(b));
+    if (b <= 32 && depth >= 6)
+      EVD_TRY(launch_chase_group<6>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride,
+                                    ticket));
+    else if (b <= 32 && depth >= 4)
+      EVD_TRY(launch_chase_group<4>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride,
+                                    ticket));
+    else if (b <= 32 && depth == 3)
+      EVD_TRY(launch_chase_group<3>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride,
+                                    ticket));
+    else if (b <= 32 && depth == 2)
+      EVD_TRY(launch_chase_group<2>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride,
+                                    ticket));
+    else if (b <= 32)
       EVD_TRY(launch_chase<32>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride, ticket));
     else if (b <= 64)
       EVD_TRY(launch_chase<64>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride, ticket));

@@ -26,6 +26,7 @@ struct Tuning {
   int sbr_back_block = 0;  // WY block width of the SBR back transformation (0 = 1024)
   int bc_valu = 0;         // 1: BC back transformation on the FP32 VALU kernel only
   int skinny_split = 0;    // split-K target (workgroups per CU) of long-K skinny products (0 = 4)
+  int chase_depth = 0;     // sweeps per workgroup of the bulge chase (0 = 1: one sweep per workgroup)
 };
 
 // Merge plan of the tridiagonal divide and conquer for one order n (host
@@ -37,6 +38,8 @@ struct DcPlan {
   std::vector<int> level_maxN;   // largest merge order of each height
   std::vector<int> level_tiles;  // grouped-GEMM tile budget of each height
   void *dev = nullptr;
+  bool own = false;              // dev is its own allocation (not carved from the plan pool)
+  std::vector<char> host;        // source of the stream-ordered upload, kept alive with the plan
 };
 
 }  // namespace evd
@@ -50,6 +53,7 @@ struct evd_ctx {
   // arena: one device allocation, bump-allocated inside a top-level call
   char *arena = nullptr;
   size_t arena_bytes = 0;
+  bool arena_external = false;  // the arena is lent by the owner (serving pool): never grown or freed here
   size_t arena_top = 0;
   size_t budget = 0;  // 0 = unlimited
   int depth = 0;      // nesting depth of public entry points
@@ -64,6 +68,10 @@ struct evd_ctx {
   int precision = EVD_PREC_FP32;
 
   std::map<long, evd::DcPlan> dc_plans;  // tridiagonal merge plans by order | (batch count << 32)
+  // device pool the plans are carved from (allocated once: the first plan,
+  // or evd::plan_pool_reserve), so a new order costs no device allocation
+  char *plan_pool = nullptr;
+  size_t plan_pool_bytes = 0, plan_pool_used = 0;
   bool capturing = false;                 // inside hipGraph capture: no host round trips
   int graphs = 0;                         // live captured graphs: the arena may not move
   long shape_div = 1;                     // GEMM launches carry this many problems' products (batched SBR)
@@ -75,6 +83,7 @@ struct evd_ctx {
   bool timing = false;
   hipEvent_t ev[EVD_STAGE_COUNT + 1] = {};
   hipEvent_t kev[2] = {};  // around the BC back-transformation kernel
+  hipEvent_t cev[2] = {};  // around the bulge-chase kernel
   float stage_ms[EVD_TIMER_COUNT] = {};
 
   char err[1024] = {0};
@@ -173,6 +182,14 @@ constexpr int kDGroupTile = 64;
 
 int max_coresident_blocks(evd_ctx *ctx, const void *kernel, int block, size_t dyn_smem, int *out);
 
+// Lend the context an arena it does not own (nullptr = none); frees an own
+// arena first.  Calls that need more than `bytes` fail with EVD_E_NOMEM.
+int ctx_lend_arena(evd_ctx *ctx, void *p, size_t bytes);
+
+// the D&C plan pool of a context (ctx.hip): allocated once with `bytes`
+int plan_pool_reserve(evd_ctx *ctx, size_t bytes);
+constexpr size_t kPlanPoolBytes = (size_t)4 << 20;  // a plan is ~100 KB at n = 16384
+
 // error word readback (synchronises the stream)
 int check_device_error(evd_ctx *ctx, const char *where);
 
@@ -180,6 +197,10 @@ int check_device_error(evd_ctx *ctx, const char *where);
 // solve will run with (0 = default block)
 size_t syevd_workspace(long n, long b, long nb, char jobz, int precision, long sbr_knob = 0);
 size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, int precision, long sbr_knob = 0);
+
+// what syevd_call / syevd_batched_call will reserve on this context
+size_t syevd_call_workspace(const evd_ctx *ctx, long n, long b, long nb, char jobz);
+size_t syevd_batched_call_workspace(const evd_ctx *ctx, int count, long n, long b, long nb, char jobz);
 
 // evd_syevd_f32 after its argument checks (syevd.hip)
 int syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, double *w, float *Z, long ldz);

@@ -27,6 +27,9 @@ int arena_ensure(evd_ctx *ctx, size_t bytes) {
     return set_error(ctx, EVD_E_UNSUPPORTED,
                      "arena of %zu bytes cannot grow to %zu while captured graphs use it; destroy them first",
                      ctx->arena_bytes, need);
+  if (ctx->arena_external)
+    return set_error(ctx, EVD_E_NOMEM, "lent arena of %zu bytes is smaller than the %zu bytes the call needs",
+                     ctx->arena_bytes, need);
   if (ctx->budget && need > ctx->budget)
     return set_error(ctx, EVD_E_NOMEM, "arena of %zu bytes exceeds the budget of %zu bytes", need, ctx->budget);
   if (ctx->arena) {
@@ -141,6 +144,29 @@ int max_coresident_blocks(evd_ctx *ctx, const void *kernel, int block, size_t dy
   return 0;
 }
 
+int ctx_lend_arena(evd_ctx *ctx, void *p, size_t bytes) {
+  if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "arena swap inside a call");
+  if (ctx->arena && !ctx->arena_external) {
+    EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EVD_HIP(ctx, hipFree(ctx->arena));
+  }
+  ctx->arena = static_cast<char *>(p);
+  ctx->arena_bytes = p ? bytes : 0;
+  ctx->arena_external = p != nullptr;
+  return 0;
+}
+
+int plan_pool_reserve(evd_ctx *ctx, size_t bytes) {
+  if (ctx->plan_pool) return 0;
+  void *p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return set_error(ctx, EVD_E_NOMEM, "plan pool of %zu bytes: %s", bytes, hipGetErrorString(e));
+  ctx->plan_pool = static_cast<char *>(p);
+  ctx->plan_pool_bytes = bytes;
+  ctx->plan_pool_used = 0;
+  return 0;
+}
+
 int check_device_error(evd_ctx *ctx, const char *where) {
   unsigned h = 0;
   EVD_HIP(ctx, hipMemcpyAsync(&h, ctx->ctrl + kCtrlError, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
@@ -207,6 +233,11 @@ int evd_ctx_create(int device, void *stream, evd_ctx **out) {
       delete ctx;
       return EVD_E_HIP;
     }
+  for (auto &e : ctx->cev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return EVD_E_HIP;
+    }
   *out = ctx;
   return 0;
 }
@@ -216,13 +247,16 @@ int evd_ctx_destroy(evd_ctx *ctx) {
   if (ctx->graphs > 0) return set_error(ctx, EVD_E_UNSUPPORTED, "%d captured graphs still use this context", ctx->graphs);
   hipStreamSynchronize(ctx->stream);
   for (auto &kv : ctx->dc_plans)
-    if (kv.second.dev) hipFree(kv.second.dev);
-  if (ctx->arena) hipFree(ctx->arena);
+    if (kv.second.dev && kv.second.own) hipFree(kv.second.dev);
+  if (ctx->plan_pool) hipFree(ctx->plan_pool);
+  if (ctx->arena && !ctx->arena_external) hipFree(ctx->arena);
   if (ctx->ctrl) hipFree(ctx->ctrl);
   if (ctx->tags) hipFree(ctx->tags);
   for (auto &e : ctx->ev)
     if (e) hipEventDestroy(e);
   for (auto &e : ctx->kev)
+    if (e) hipEventDestroy(e);
+  for (auto &e : ctx->cev)
     if (e) hipEventDestroy(e);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -280,7 +314,8 @@ int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value) {
     int lo, hi;
   } knobs[] = {{"bc_sweeps", &ctx->tune.bc_sweeps, 0, 64},          {"bc_waves", &ctx->tune.bc_waves, 0, 16},
                {"chase_groups", &ctx->tune.chase_groups, 0, 1 << 20}, {"sbr_back_block", &ctx->tune.sbr_back_block, 0, 1 << 16},
-               {"bc_valu", &ctx->tune.bc_valu, 0, 1},           {"skinny_split", &ctx->tune.skinny_split, 0, 4}};
+               {"bc_valu", &ctx->tune.bc_valu, 0, 1},           {"skinny_split", &ctx->tune.skinny_split, 0, 4},
+               {"chase_depth", &ctx->tune.chase_depth, 0, 6}};
   for (auto &kb : knobs)
     if (!strcmp(kb.name, name)) {
       if (value < kb.lo || value > kb.hi) return -3;

@@ -10,7 +10,8 @@
 // are [v; 0]: w(A) = w(P)[0:n] and Z(A) = Z(P)[0:n, 0:n].  Every bucket (and
 // eigenvector flag) is one evd_syevd_batched_f32 solve, i.e. one launch per
 // stage for all of its problems.  Orders above the largest bucket are solved
-// one at a time, unpadded.
+// one at a time, unpadded.  With b = 0 each bucket of order N runs at
+// bandwidth min(32, N / 4) (16 and 24 for the two smallest buckets).
 #include "common.hpp"
 
 namespace evd {
@@ -108,7 +109,7 @@ extern "C" int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float
   if (!ctx) return -1;
   EVD_ARG(ctx, count >= 0, 2);
   EVD_ARG(ctx, count == 0 || (n && A && lda && w && Z && ldz && jobz), 3);
-  EVD_ARG(ctx, b == 32 || b == 64, 10);
+  EVD_ARG(ctx, b == 0 || b == 32 || b == 64, 10);  // 0: b = min(32, N / 4) for each bucket order N
   if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_mixed_f32 cannot nest");
   for (int q = 0; q < count; q++) {
     const bool vec = (jobz[q] == 'V' || jobz[q] == 'v');
@@ -125,7 +126,7 @@ extern "C" int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float
       alone.push_back(q);
       continue;
     }
-    const long N = bucket_order(std::max<long>(n[q], 2 * b));
+    const long N = bucket_order(std::max<long>(n[q], b > 0 ? 2 * b : 64));
     auto it = std::find_if(groups.begin(), groups.end(), [&](const Group &g) { return g.N == N && g.jobz == jz; });
     if (it == groups.end()) {
       groups.push_back(Group{N, jz, {}});
@@ -137,13 +138,14 @@ extern "C" int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float
     const int c = (int)g.members.size();
     const long N = g.N;
     const bool vec = g.jobz == 'V';
-    const long nb = b * std::max<long>(1, 1024 / b);
+    const long bg = b > 0 ? b : std::min<long>(32, N / 4);  // the bucket's bandwidth
+    const long nb = bg * std::max<long>(1, 1024 / bg);
     // staging: per-problem descriptors, norms, padded matrices and results
     const size_t desc = align256((size_t)c * sizeof(long)) * 3 + align256((size_t)c * sizeof(void *)) * 3 +
                         align256((size_t)c * sizeof(double));
     const size_t stage = desc + align256((size_t)c * N * N * sizeof(float)) * (vec ? 2 : 1) +
                          align256((size_t)c * N * sizeof(double));
-    EVD_TRY(arena_ensure(ctx, stage + syevd_batched_workspace(c, N, b, nb, g.jobz, ctx->precision)));
+    EVD_TRY(arena_ensure(ctx, stage + syevd_batched_workspace(c, N, bg, nb, g.jobz, ctx->precision)));
     ArenaFrame frame(ctx);
     long *dn = arena_alloc_n<long>(ctx, c), *dlda = arena_alloc_n<long>(ctx, c), *dldz = arena_alloc_n<long>(ctx, c);
     float **dA = arena_alloc_n<float *>(ctx, c), **dZ = arena_alloc_n<float *>(ctx, c);
@@ -189,10 +191,10 @@ extern "C" int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float
       AmaxRows(evd_ctx *c_, const long *rows) : c(c_) { c->amax_rows = rows; }
       ~AmaxRows() { c->amax_rows = nullptr; }
     } amax_rows(ctx, hn.data());
-    EVD_TRY(syevd_batched_call(ctx, g.jobz, c, N, b, nb, P, N, N * N, wP, N, ZP, N, N * N));
+    EVD_TRY(syevd_batched_call(ctx, g.jobz, c, N, bg, nb, P, N, N * N, wP, N, ZP, N, N * N));
     unpad_kernel<<<grid, 256, 0, ctx->stream>>>(N, dn, wP, ZP, dw, dZ, dldz);
     EVD_LAUNCHED(ctx);
   }
-  for (int q : alone) EVD_TRY(evd_syevd_f32(ctx, jobz[q], n[q], b, 0, A[q], lda[q], w[q], Z[q], ldz[q]));
+  for (int q : alone) EVD_TRY(evd_syevd_f32(ctx, jobz[q], n[q], b > 0 ? b : 32, 0, A[q], lda[q], w[q], Z[q], ldz[q]));
   return check_device_error(ctx, "evd_syevd_mixed_f32");
 }

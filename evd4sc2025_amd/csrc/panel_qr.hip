@@ -63,11 +63,11 @@ constexpr unsigned long long panel_sentinel(unsigned epoch, int step) {
 // short branch-free loop (a runtime column index would force select chains
 // or private memory, and a fully unrolled column loop a code footprint far
 // beyond the instruction cache).
-template <int NB, int RPT>
-__global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const float *__restrict__ Ain, long lda,
-                                                       float *__restrict__ Y, long ldy, float *__restrict__ W,
-                                                       long ldw, float *__restrict__ R, long ldr,
-                                                       float *__restrict__ Aout, long ldo, double *__restrict__ part,
+template <int NB, int RPT, typename T>
+__global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const T *__restrict__ Ain, long lda,
+                                                       T *__restrict__ Y, long ldy, T *__restrict__ W,
+                                                       long ldw, T *__restrict__ R, long ldr,
+                                                       T *__restrict__ Aout, long ldo, double *__restrict__ part,
                                                        double *__restrict__ tau_out, unsigned *__restrict__ tags,
                                                        unsigned *__restrict__ epoch_word, unsigned *__restrict__ err,
                                                        long s_in, long s_y, long s_w, long s_out) {
@@ -359,8 +359,8 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
       for (int c = 0; c < NB; c++) {
         if (c >= nc) continue;
         const double rv = (r <= c) ? a[q][c] : 0.0;
-        if (R && r < nc) R[r + (long)c * ldr] = (float)rv;
-        if (Aout) Aout[r + (long)c * ldo] = (float)rv;
+        if (R && r < nc) R[r + (long)c * ldr] = (T)rv;
+        if (Aout) Aout[r + (long)c * ldo] = (T)rv;
       }
     }
 #pragma unroll
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
     if (r < m && Y) {
 #pragma unroll
       for (int c = 0; c < NB; c++)
-        if (c < nc) Y[r + (long)c * ldy] = (float)a[q][c];
+        if (c < nc) Y[r + (long)c * ldy] = (T)a[q][c];
     }
   }
   if (W) {
@@ -379,21 +379,21 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const floa
         double w = 0.0;
 #pragma unroll
         for (int l = 0; l < NB; l++) w += a[q][l] * Tt[c][l];
-        if (r < m) W[r + (long)c * ldw] = (float)w;
+        if (r < m) W[r + (long)c * ldw] = (T)w;
       }
     }
   }
 }
 
-template <int NB, int RPT>
-static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, long ldy, float *W,
-                        long ldw, float *R, long ldr, float *Aout, long ldo, double *tau_out, int count = 1,
-                        long s_in = 0, long s_y = 0, long s_w = 0, long s_out = 0) {
+template <int NB, int RPT, typename T>
+static int launch_panel(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *Y, long ldy, T *W, long ldw, T *R,
+                        long ldr, T *Aout, long ldo, double *tau_out, int count = 1, long s_in = 0, long s_y = 0,
+                        long s_w = 0, long s_out = 0) {
   const long rows_per = 256L * RPT;
   int G = (int)((m + rows_per - 1) / rows_per);
   if (count > 1 && (R || tau_out)) return set_error(ctx, EVD_E_UNSUPPORTED, "batched panels write no R / tau output");
   int maxres = 0;
-  EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<NB, RPT>, 256, 0, &maxres));
+  EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<NB, RPT, T>, 256, 0, &maxres));
   if ((long)G * count > maxres)
     return set_error(ctx, EVD_E_UNSUPPORTED, "%d panels of %ld rows need %ld co-resident groups (max %d)", count, m,
                      (long)G * count, maxres);
@@ -411,21 +411,31 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const float *A, long lda,
   // one workgroup is trivially co-resident: a regular launch, which (unlike a
   // cooperative one) may overlap with the work of other streams
   if (G == 1)
-    EVD_HIP(ctx, hipLaunchKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G, count), dim3(256), args, 0,
+    EVD_HIP(ctx, hipLaunchKernel((const void *)panel_qr_kernel<NB, RPT, T>, dim3(G, count), dim3(256), args, 0,
                                  ctx->stream));
   else
-    EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT>, dim3(G, count), dim3(256), args,
+    EVD_HIP(ctx, hipLaunchCooperativeKernel((const void *)panel_qr_kernel<NB, RPT, T>, dim3(G, count), dim3(256), args,
                                             0, ctx->stream));
   return 0;
 }
 
-// Panel factorisation used by the SBR driver and by evd_panel_qr_f32.
+// Panel factorisation used by the SBR drivers (FP32 and FP64 storage; the
+// arithmetic is FP64 in both) and by evd_panel_qr_f32 / _f64.
+template <typename T>
+int panel_qr_t(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *Y, long ldy, T *W, long ldw, T *R, long ldr,
+               T *Aout, long ldo, double *tau_out) {
+  if (m <= 0 || nc <= 0) return 0;
+  if (nc <= 32) return launch_panel<32, 2, T>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
+  if (nc <= 64) return launch_panel<64, 1, T>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
+  return set_error(ctx, EVD_E_UNSUPPORTED, "panel width %ld > 64", nc);
+}
 int panel_qr(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, long ldy, float *W, long ldw,
              float *R, long ldr, float *Aout, long ldo, double *tau_out) {
-  if (m <= 0 || nc <= 0) return 0;
-  if (nc <= 32) return launch_panel<32, 2>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
-  if (nc <= 64) return launch_panel<64, 1>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
-  return set_error(ctx, EVD_E_UNSUPPORTED, "panel width %ld > 64", nc);
+  return panel_qr_t<float>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
+}
+int panel_qr_f64(evd_ctx *ctx, long m, long nc, const double *A, long lda, double *Y, long ldy, double *W, long ldw,
+                 double *R, long ldr, double *Aout, long ldo, double *tau_out) {
+  return panel_qr_t<double>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
 }
 
 // Same-size panels of `count` problems (A, Y, W of problem q at q times the
@@ -440,17 +450,17 @@ int panel_qr_batched(evd_ctx *ctx, int count, long m, long nc, float *A, long ld
   int maxres = 0;
   if (count > 1 && G > 1) {
     if (nc <= 32)
-      EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<32, 2>, 256, 0, &maxres));
+      EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<32, 2, float>, 256, 0, &maxres));
     else
-      EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<64, 1>, 256, 0, &maxres));
+      EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<64, 1, float>, 256, 0, &maxres));
   }
   if (count > 1 && nc <= 64 && (G == 1 || (G * count <= maxres && G * count <= kTagWords))) {
     if (nc <= 32)
-      return launch_panel<32, 2>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA, sY, sW,
-                                 sA);
+      return launch_panel<32, 2, float>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA,
+                                        sY, sW, sA);
     if (nc <= 64)
-      return launch_panel<64, 1>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA, sY, sW,
-                                 sA);
+      return launch_panel<64, 1, float>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA,
+                                        sY, sW, sA);
   }
   for (int q = 0; q < count; q++)
     EVD_TRY(panel_qr(ctx, m, nc, A + q * sA, lda, Y + q * sY, ldy, W + q * sW, ldw, nullptr, 0, A + q * sA, lda,

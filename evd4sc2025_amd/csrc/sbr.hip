@@ -232,7 +232,9 @@ long sbr_back_blk(long knob, long b, long wcols) {
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols) {
   const long blk = sbr_back_blk(ctx->tune.sbr_back_block, b, wcols);
-  if (wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
+  // a block of b (no merge; b not dividing the padding, e.g. b = 24) is applied panel by panel
+  if (blk != b && wcols % blk != 0)
+    return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
   EVD_TRY(wy_merge_doubling(ctx, n, b, blk, wcols, W, ldw, Y, ldy));
   return sbr_apply(ctx, n, b, blk, wcols, ncols, Z, ldz, W, ldw, Y, ldy);
 }
@@ -244,7 +246,7 @@ int sbr_back_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float 
                      long ldw, const float *Y, long ldy, long wcols) {
   if (count == 1) return sbr_back(ctx, n, b, ncols, Z, ldz, W, ldw, Y, ldy, wcols);
   const long blk = sbr_back_blk(ctx->tune.sbr_back_block, b, wcols);
-  if (wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
+  if (blk != b && wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
   ArenaFrame frame(ctx);
   struct ShapeDiv {
     evd_ctx *c;

@@ -825,14 +825,26 @@ int dc_plan(evd_ctx *ctx, long n, int count, const DcPlan **out) {
   for (const MergeDesc &m : all) split[m.start + m.n1] = 1;
   const size_t b_md = align256(sizeof(MergeDesc) * all.size()), b_split = align256(sizeof(int) * nt),
                b_tiles = align256(sizeof(int) * tiles.size());
-  std::vector<char> host(b_md + b_split + b_tiles, 0);
+  std::vector<char> &host = plan.host;
+  host.assign(b_md + b_split + b_tiles, 0);
   memcpy(host.data(), all.data(), sizeof(MergeDesc) * all.size());
   memcpy(host.data() + b_md, split.data(), sizeof(int) * nt);
   memcpy(host.data() + b_md + b_split, tiles.data(), sizeof(int) * tiles.size());
-  EVD_HIP(ctx, hipMalloc(&plan.dev, host.size()));
-  hipError_t e = hipMemcpy(plan.dev, host.data(), host.size(), hipMemcpyHostToDevice);
+  // device copy: carved from the context's plan pool (one allocation for all
+  // plans), else an allocation of its own; the upload is stream-ordered and
+  // its source stays alive with the plan
+  EVD_TRY(plan_pool_reserve(ctx, kPlanPoolBytes));
+  const size_t need = align256(host.size());
+  if (ctx->plan_pool_used + need <= ctx->plan_pool_bytes) {
+    plan.dev = ctx->plan_pool + ctx->plan_pool_used;
+    ctx->plan_pool_used += need;
+  } else {
+    EVD_HIP(ctx, hipMalloc(&plan.dev, host.size()));
+    plan.own = true;
+  }
+  const hipError_t e = hipMemcpyAsync(plan.dev, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) {
-    hipFree(plan.dev);
+    if (plan.own) hipFree(plan.dev);
     return set_error(ctx, EVD_E_HIP, "D&C plan upload failed: %s", hipGetErrorString(e));
   }
   auto ins = ctx->dc_plans.emplace(key, std::move(plan));

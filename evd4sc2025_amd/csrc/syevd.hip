@@ -39,6 +39,8 @@ bool bc_back_wy_fits(long n, long ldq);
 int bc_back_wy_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride, const float *U,
                        long ldu, long u_stride, bool split16);
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
+int bc_back_small_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float *Q, long ldq, long q_stride,
+                          const float *U, long ldu, long u_stride, bool rev);
 int sbr_back(evd_ctx *ctx, long n, long b, long ncols, float *Z, long ldz, float *W, long ldw, const float *Y,
              long ldy, long wcols);
 size_t sy2sb_batched_workspace(int count, long n, long b, long nb);
@@ -252,9 +254,11 @@ int syevd(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long lda, 
       hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]);
       ctx->stage_ms[i] = ms;
     }
-    float km = 0.f;
+    float km = 0.f, cm = 0.f;
     if (vec) hipEventElapsedTime(&km, ctx->kev[0], ctx->kev[1]);
     ctx->stage_ms[EVD_TIMER_BCBACK_KERNEL] = km;
+    if (n > 2) hipEventElapsedTime(&cm, ctx->cev[0], ctx->cev[1]);
+    ctx->stage_ms[EVD_TIMER_CHASE_KERNEL] = cm;
   }
   return 0;
 }
@@ -330,7 +334,9 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
       EVD_LAUNCHED(ctx);
     }
     stage_mark(ctx, 3);
-    if (b == 32 && !ctx->tune.bc_valu && bc_back_wy_fits(n, ldz)) {
+    if (b != 32 && b != 64) {
+      EVD_TRY(bc_back_small_batched(ctx, count, n, b, n, Z, ldz, z_stride, U, ldu, u_stride, true));
+    } else if (b == 32 && !ctx->tune.bc_valu && bc_back_wy_fits(n, ldz)) {
       EVD_TRY(bc_back_wy_batched(ctx, count, n, n, Z, ldz, z_stride, U, ldu, u_stride,
                                  ctx->precision == EVD_PREC_FP16));
     } else {
@@ -353,9 +359,11 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
       hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]);
       ctx->stage_ms[i] = ms;
     }
-    float km = 0.f;
+    float km = 0.f, cm = 0.f;
     if (vec) hipEventElapsedTime(&km, ctx->kev[0], ctx->kev[1]);
     ctx->stage_ms[EVD_TIMER_BCBACK_KERNEL] = km;
+    if (n > 2) hipEventElapsedTime(&cm, ctx->cev[0], ctx->cev[1]);
+    ctx->stage_ms[EVD_TIMER_CHASE_KERNEL] = cm;
   }
   return 0;
 }
@@ -415,7 +423,8 @@ extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, f
   const bool vec = (jobz == 'V' || jobz == 'v');
   EVD_ARG(ctx, vec || jobz == 'N' || jobz == 'n', 2);
   EVD_ARG(ctx, n >= 0, 3);
-  EVD_ARG(ctx, b == 32 || b == 64 || (!vec && b >= 1 && b <= 64), 4);
+  // eigenvectors with b other than 32 / 64: orders up to 128 (small-order back transformation)
+  EVD_ARG(ctx, b == 32 || b == 64 || (b >= 1 && b <= 64 && (!vec || (b >= 2 && n <= 128))), 4);
   EVD_ARG(ctx, nb == 0 || (nb >= b && nb % b == 0), 5);
   EVD_ARG(ctx, A != nullptr || n == 0, 6);
   EVD_ARG(ctx, lda >= std::max<long>(1, n), 7);
@@ -424,6 +433,26 @@ extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, f
   EVD_ARG(ctx, !vec || ldz >= std::max<long>(1, n), 10);
   if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_f32 cannot nest");
   return syevd_call(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
+}
+
+// The arena bytes syevd_call / syevd_batched_call will ask for on this
+// context (the same knob and nb resolution as the calls).
+static void resolve_knobs(const evd_ctx *ctx, long n, long b, long *nb, long *sbr_knob) {
+  int tnb = 0;
+  Tuning t;
+  const bool hit = ctx->tune_cache && tune_lookup(n, b, ctx->precision, &tnb, &t);
+  *sbr_knob = ctx->tune.sbr_back_block ? ctx->tune.sbr_back_block : (hit ? t.sbr_back_block : 0);
+  if (*nb == 0) *nb = (hit && tnb > 0) ? tnb : b * std::max<long>(1, 1024 / b);
+}
+size_t evd::syevd_call_workspace(const evd_ctx *ctx, long n, long b, long nb, char jobz) {
+  long sbr = 0;
+  resolve_knobs(ctx, n, b, &nb, &sbr);
+  return syevd_workspace(n, b, nb, jobz, ctx->precision, sbr);
+}
+size_t evd::syevd_batched_call_workspace(const evd_ctx *ctx, int count, long n, long b, long nb, char jobz) {
+  long sbr = 0;
+  resolve_knobs(ctx, n, b, &nb, &sbr);
+  return syevd_batched_workspace(count, n, b, nb, jobz, ctx->precision, sbr);
 }
 
 // evd_syevd_f32 after argument checks: tuning knobs, workspace, solve.
@@ -458,7 +487,7 @@ extern "C" int evd_syevd_batched_f32(evd_ctx *ctx, char jobz, int count, long n,
   // count <= 65535: per-problem kernels carry the problem index in grid y / z
   EVD_ARG(ctx, count >= 0 && count <= 65535, 3);
   EVD_ARG(ctx, n >= 0 && (long)count * n < (1L << 30), 4);
-  EVD_ARG(ctx, b == 32 || b == 64 || (!vec && b >= 1 && b <= 64), 5);
+  EVD_ARG(ctx, b == 32 || b == 64 || (b >= 1 && b <= 64 && (!vec || (b >= 2 && n <= 128))), 5);
   EVD_ARG(ctx, nb == 0 || (nb >= b && nb % b == 0), 6);
   EVD_ARG(ctx, A != nullptr || n == 0 || count == 0, 7);
   EVD_ARG(ctx, lda >= std::max<long>(1, n), 8);
@@ -506,6 +535,25 @@ extern "C" int evd_orthogonality_f32(evd_ctx *ctx, long m, long n, const float *
   f32_to_f64_kernel<<<grid2d(m, n), 256, 0, ctx->stream>>>(m, n, Q, ldq, Q64, m);
   EVD_LAUNCHED(ctx);
   EVD_TRY(dgemm(ctx, 'T', 'N', n, n, m, 1.0, Q64, m, Q64, m, 0.0, G, n));
+  double s = 0.0;
+  EVD_TRY(frob_sq(ctx, n, n, G, n, true, acc, &s));
+  *out = sqrt(s) / (double)(n > 0 ? n : 1);
+  return 0;
+}
+
+extern "C" int evd_orthogonality_f64(evd_ctx *ctx, long m, long n, const double *Q, long ldq, double *out) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, m >= 0, 2);
+  EVD_ARG(ctx, n >= 0, 3);
+  EVD_ARG(ctx, ldq >= std::max<long>(1, m), 5);
+  EVD_ARG(ctx, out != nullptr, 6);
+  if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "metrics cannot nest");
+  EVD_TRY(arena_ensure(ctx, align256((size_t)n * n * 8) + 4096));
+  ArenaFrame frame(ctx);
+  double *G = arena_alloc_n<double>(ctx, (size_t)n * n);
+  double *acc = arena_alloc_n<double>(ctx, 1);
+  if (!G || !acc) return EVD_E_NOMEM;
+  EVD_TRY(dgemm(ctx, 'T', 'N', n, n, m, 1.0, Q, ldq, Q, ldq, 0.0, G, n));
   double s = 0.0;
   EVD_TRY(frob_sq(ctx, n, n, G, n, true, acc, &s));
   *out = sqrt(s) / (double)(n > 0 ? n : 1);

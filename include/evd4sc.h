@@ -47,10 +47,13 @@ typedef struct evd_ctx evd_ctx;
 #define EVD_STAGE_BCBACK 3
 #define EVD_STAGE_SBRBACK 4
 #define EVD_STAGE_COUNT 5
-/* extra timer of evd_ctx_stage_ms(): the BC back-transformation kernel alone
- * (the stage time EVD_STAGE_BCBACK also covers packing its reflectors) */
+/* extra timers of evd_ctx_stage_ms(): the BC back-transformation kernel alone
+ * (the stage time EVD_STAGE_BCBACK also covers packing its reflectors) and
+ * the bulge-chase kernel alone (EVD_STAGE_BC also covers the band packing
+ * and the extraction of D and E) */
 #define EVD_TIMER_BCBACK_KERNEL 5
-#define EVD_TIMER_COUNT 6
+#define EVD_TIMER_CHASE_KERNEL 6
+#define EVD_TIMER_COUNT 7
 
 /* ---- context ---------------------------------------------------------- */
 
@@ -84,7 +87,8 @@ int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value);
 int evd_ctx_set_precision(evd_ctx *ctx, int precision);
 /* Per-stage device time (ms) of the last evd_syevd_f32 call, measured with
  * hipEvents on the context stream; enable with evd_ctx_enable_timing.  count
- * may be up to EVD_TIMER_COUNT (ms[5] = the BC back-transformation kernel). */
+ * may be up to EVD_TIMER_COUNT (ms[5] = the BC back-transformation kernel,
+ * ms[6] = the bulge-chase kernel). */
 int evd_ctx_enable_timing(evd_ctx *ctx, int on);
 int evd_ctx_stage_ms(const evd_ctx *ctx, float *ms, int count);
 int evd_ctx_synchronize(evd_ctx *ctx);
@@ -195,6 +199,33 @@ size_t evd_syevd_workspace_f32(long n, long b, long nb, char jobz);
  * its WY blocks, 16 KiB per 32 x 32 reflector block, in the arena). */
 size_t evd_syevd_workspace_ex(long n, long b, long nb, char jobz, int precision);
 
+/* ---- FP64 forms of the reference entry points ---------------------------
+ * The reference's api/ library is instantiated for double only
+ * (api/zy_zy_sy2sb.cu:488-507 and the double signatures of
+ * api/inc/computerWYFromSlide.h:4-5, api/inc/computerQFromWY.h:4-5,
+ * api/inc/BC_backTrans.h:3); these take its double buffers as they are and
+ * compute in FP64 (FP64 matrix cores for the GEMMs).  Arguments and results
+ * as the _f32 forms above. */
+int evd_panel_qr_f64(evd_ctx *ctx, long m, long nb, double *A, long lda, double *W, long ldw, double *R, long ldr);
+/* The explicit thin Q of hou_tsqr_panel<double> (api/inc/TallShinnyQR.h:9-134). */
+int evd_orgqr_f64(evd_ctx *ctx, long m, long n, double *Q, long ldq, const double *W, long ldw, double *work);
+/* Replaces tc_ozimmu_syr2k<double> (api/TCblas/tc_ozimmu_syr2k.cu:209-341). */
+int evd_syr2k_f64(evd_ctx *ctx, long n, long k, double alpha, const double *A, long lda, const double *B, long ldb,
+                  double beta, double *C, long ldc);
+/* Replaces my_ZY_ZY_SBR_Vector<double> (api/zy_zy_sy2sb.cu:24-486). */
+int evd_sy2sb_f64(evd_ctx *ctx, long n, long b, long nb, double *A, long lda, double *W, long ldw, double *Y,
+                  long ldy);
+/* Replaces computerWYFromSlide (api/computerWYFromSlide.cu:5-25). */
+int evd_wy_from_slide_f64(evd_ctx *ctx, long M, long N, long slide, double *W, long ldw, const double *Y, long ldy);
+/* Replaces computerQFromWY (api/computerQFromWY.cu:5-18). */
+int evd_q_from_wy_f64(evd_ctx *ctx, long M, long N, double *Q, long ldq, const double *W, long ldw, const double *Y,
+                      long ldy);
+/* Replaces my_BC_back_trans_v8_10_noBandU (api/BC_backTransV8_10_noBandU.cu:
+ * 234-313; order 0 is its product, order 1 the reverse).  b must be 32, the
+ * bandwidth the reference's kernel is written for. */
+int evd_bc_back_f64(evd_ctx *ctx, long n, long b, long ncols, double *Q, long ldq, const double *U, long ldu,
+                    int order);
+
 /* ---- batched solves (no counterpart in the reference, which solves one
  * matrix per process) ----
  * count same-size problems in strided storage: A + q a_stride (overwritten),
@@ -219,7 +250,8 @@ int evd_sb2st_batched_f64(evd_ctx *ctx, int count, long n, long b, double *band,
  * the next bucket order (64, 96, 128, 192, ... up to 4096) with a diagonal pad
  * above ||A||_inf, and each (bucket, jobz) group is one batched solve; larger
  * orders are solved alone.  Results equal an unpadded solve within rounding
- * (not bit for bit).  Synchronises the stream. */
+ * (not bit for bit).  b = 32 or 64, or 0: each bucket of order N at bandwidth
+ * min(32, N / 4).  Synchronises the stream. */
 int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float *const *A, const long *lda, double *const *w,
                         float *const *Z, const long *ldz, const char *jobz, long b);
 
@@ -272,6 +304,8 @@ int evd_ctx_use_tune_cache(evd_ctx *ctx, int on);
 
 /* ||Q^T Q - I||_F / n, computed in FP64 on the GPU. */
 int evd_orthogonality_f32(evd_ctx *ctx, long m, long n, const float *Q, long ldq, double *out);
+/* The same for an FP64 Q (the reference's checkOrthogonality takes double). */
+int evd_orthogonality_f64(evd_ctx *ctx, long m, long n, const double *Q, long ldq, double *out);
 /* ||A Z - Z diag(w)||_F / (n ||A||_F), FP64 on the GPU (A full symmetric). */
 int evd_backward_error_f32(evd_ctx *ctx, long n, const float *A, long lda, const double *w, const float *Z, long ldz,
                            double *out);
@@ -299,10 +333,12 @@ int evd_gen_spectrum_f32(evd_ctx *ctx, long n, unsigned long long seed, const do
 /* ---------------------------------------------------------------- serving
  * Request loop for eigenproblems of uneven sizes.  No counterpart in the
  * reference, whose driver solves one matrix per process run
- * (src/EVD/myEVD.cu:251-706).  `slots` contexts, each with its own stream and
- * device-resident workspace arena, run admitted requests concurrently; a
- * request is admitted (FIFO) when a slot is free and the slot arenas, grown to
- * its workspace, stay within hbm_budget bytes (0 = unlimited).  A, w, Z are
+ * (src/EVD/myEVD.cu:251-706).  `slots` contexts, each with its own stream,
+ * run admitted requests concurrently.  With hbm_budget > 0 the budget is one
+ * device allocation made here; each admitted request (or same-size batch)
+ * borrows a contiguous region of it as its workspace and returns it when it
+ * completes, so nothing is allocated or freed while requests are in flight
+ * (hbm_budget = 0: unlimited, each slot grows its own workspace).  A, w, Z are
  * the caller's device buffers (A column-major n x n FP32, lower triangle read
  * and overwritten; w n FP64; Z n x n FP32 or NULL for eigenvalues only): A must
  * be complete when submitted (the slots run on their own non-blocking streams,
@@ -317,6 +353,11 @@ int evd_server_destroy(evd_server *srv);
  * that fit the HBM budget together and solves them as one
  * evd_syevd_batched_f32 call (bitwise the results of single solves). */
 int evd_server_set_batching(evd_server *srv, int on); /* finishes queued requests first */
+/* Admission order: 0 (default) = the queued request of the largest order
+ * first (FIFO among equal orders), 1 = FIFO.  Largest-first starts the
+ * slowest requests as soon as a slot is free, which shortens the latency tail
+ * of a mix of orders. */
+int evd_server_set_policy(evd_server *srv, int policy);
 /* GEMM precision of every slot (EVD_PREC_*, as evd_ctx_set_precision); only
  * while no request is queued or running (else EVD_E_UNSUPPORTED). */
 int evd_server_set_precision(evd_server *srv, int precision);

@@ -254,3 +254,28 @@ def test_gaussian_generator_matches_oracle(ctx, n, seed):
     B = O.gen_symmetric(n, seed, 1).astype(np.float64)
     assert np.all(np.abs(A - B) <= 2.0 ** -23 * np.abs(B) + 1e-30)
     assert np.mean(A == B) > 0.99
+
+
+@pytest.mark.parametrize("n,b,depth", [(600, 32, 3), (1500, 32, 3), (1500, 32, 2), (130, 32, 3), (40, 32, 3),
+                                       (5, 32, 3), (3, 32, 3), (700, 16, 6), (700, 16, 4), (500, 20, 3),
+                                       (333, 8, 6), (2000, 32, 1)])
+def test_sb2st_sweep_depth_matches_oracle(ctx, n, b, depth):
+    """The grouped chase (depth sweeps per workgroup, handed over in LDS) and
+    the one-sweep-per-workgroup chase (depth 1) against the oracle."""
+    import evd4sc2025_amd as E
+
+    print(f"chase n={n} b={b} depth={depth}", flush=True)
+    A = O.gen_symmetric(n, 500 + n + b, 0).astype(np.float64)
+    Ab, _, _ = O.sy2sb(A, b, 2 * b)
+    B32 = O.band_of(Ab, b).astype(np.float32).astype(np.float64)
+    band = E.band_pack(ctx, to_dev(B32.astype(np.float32)), b)
+    ctx.set_knob("chase_depth", depth)
+    try:
+        D, Ee, U = E.sb2st(ctx, band, n, b)
+    finally:
+        ctx.set_knob("chase_depth", 0)
+    D, Ee, U = to_host(D), to_host(Ee), to_host(U)
+    D1, E1, U1 = O.sb2st(B32, b)
+    np.testing.assert_allclose(D, D1, atol=1e-10)
+    np.testing.assert_allclose(Ee, E1, atol=1e-10)
+    np.testing.assert_allclose(U, U1, atol=1e-5)

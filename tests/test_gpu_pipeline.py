@@ -87,9 +87,9 @@ def test_syevd_known_spectrum(ctx):
 def test_syevd_rejects_bad_arguments(ctx):
     import evd4sc2025_amd as E
 
-    dA = E.gen_symmetric(ctx, 64, 0, 0)
+    dA = E.gen_symmetric(ctx, 200, 0, 0)
     with pytest.raises(E.EvdError) as ei:
-        E.syevd(ctx, dA, b=48, nb=96)  # b must be 32 or 64 for vectors
+        E.syevd(ctx, dA, b=48, nb=96)  # b must be 32 or 64 for vectors above order 128
     assert ei.value.code == -4
     with pytest.raises(E.EvdError) as ei:
         E.syevd(ctx, dA, b=32, nb=48)  # nb % b != 0

@@ -121,3 +121,35 @@ def test_server_fp16_precision_matches_single_fp16_solves():
         if jobz == "V":
             assert np.array_equal(Z, to_host(r.Z)), seed
     ctx.close()
+
+
+def test_server_admission_waits_for_memory_without_hanging():
+    """A budget that holds three eigenvalue-only requests but not a fourth,
+    larger request at the same time: the larger one is admitted once memory
+    is returned (the budget is one pool lent out and returned per request),
+    and every request completes within a bounded wait."""
+    import threading
+
+    import evd4sc2025_amd as E
+
+    wsn = E.syevd_workspace(1024, 32, 256, "N")
+    wsv = E.syevd_workspace(1024, 32, 256, "V")
+    budget = 3 * wsn + (4 << 20) * 4 + (16 << 20)  # three N at once, plus the slots' plan pools
+    assert wsv < budget < 3 * wsn + wsv + (4 << 20) * 4
+    srv = E.Server(0, slots=4, hbm_budget=budget, batching=False, policy=1)
+    reqs = [(1024, "N", 60), (1024, "N", 61), (1024, "N", 62), (1024, "V", 63)]
+    done = {}
+
+    def run():
+        done["res"] = _solve_all(srv, reqs)
+
+    th = threading.Thread(target=run)
+    th.start()
+    th.join(timeout=120)
+    assert not th.is_alive(), "admission waited forever"
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 4 and st["arena_bytes"] <= budget
+    for (w, Z), (n, jobz, seed) in zip(done["res"], reqs):
+        wr = np.linalg.eigvalsh(O.gen_symmetric(n, seed, 0).astype(np.float64))
+        assert np.abs(np.sort(w) - wr).max() < 1e-5 * np.abs(wr).max()
