@@ -103,7 +103,7 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     mkl = _mkl()
     threads = host_threads()
     if mkl is None:
-        return None
+        return {"value": None, "error": "no MKL runtime (libmkl_rt) on this host"}
     mkl.MKL_Set_Num_Threads(ctypes.c_int(threads))
     n = D.shape[0]
     d = np.array(D, dtype=np.float64)
@@ -115,7 +115,7 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     t_stedc = time.perf_counter() - t0
     del Z
     if rc != 0:
-        return None
+        return {"value": None, "error": f"LAPACKE_dstedc returned {rc} at n={n}"}
     if full:
         A = np.array(A_host, dtype=np.float64, order="F")
         nn = n
@@ -129,7 +129,7 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     rc = mkl.LAPACKE_dsyevd(102, ctypes.c_char(b"V"), ctypes.c_char(b"L"), nn, dp(A), nn, dp(w))
     t_syevd = time.perf_counter() - t0
     if rc != 0:
-        return None
+        return {"value": None, "error": f"LAPACKE_dsyevd returned {rc} at n={nn}"}
     which = "the headline matrix" if full else f"the n={nn} matrix of the same workload (uniform[-1,1], seed 0)"
     return {
         "value": (10.0 / 3.0) * nn ** 3 / t_syevd / 1e12,

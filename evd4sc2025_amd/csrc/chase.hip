@@ -257,7 +257,11 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
 // sequential chase.  Groups are handed out by ticket in increasing order, so
 // a group's predecessor is always running; every wait is bounded.
 constexpr int kGW = 256;     // window columns
-constexpr int kGLD = 65;     // doubles per window column (2b <= 64 offsets + 1)
+// doubles per window column (2b <= 64 offsets).  Element (row R, column C)
+// sits at C * kGLD + R - C, so a band row runs through the window with stride
+// kGLD - 1 = 63 doubles (odd: the 32 lanes of a row access hit 32 distinct
+// banks) and a band column with stride 1.
+constexpr int kGLD = 64;
 // most columns the front brings in at one step: b + 2 at b = 32 (the first
 // chunk s0 .. n of a sweep whose first step is also its last)
 constexpr int kGChunk = 34;
@@ -381,7 +385,7 @@ __global__ __launch_bounds__((D + 1) * 64, 1) void chase_group_kernel(
             }
           }
           // ---- S <- H S H, N <- N H (S from the lower band, row ri, columns kk)
-          const int ri = lane >> 1, kc = lane & 1;
+          const int ri = lane & 31, kc = lane >> 5;
           double Sr[16], Nr[16];
 #pragma unroll
           for (int r = 0; r < 16; r++) {
@@ -397,8 +401,8 @@ __global__ __launch_bounds__((D + 1) * 64, 1) void chase_group_kernel(
             yp += Sr[r] * uk;
             qp += Nr[r] * uk;
           }
-          yp += __shfl_xor(yp, 1, 64);
-          qp += __shfl_xor(qp, 1, 64);
+          yp += __shfl_xor(yp, 32, 64);
+          qp += __shfl_xor(qp, 32, 64);
           if (kc == 0) {
             yv[ri] = yp;
             qv[ri] = qp;
