@@ -240,23 +240,29 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
 //  * it brings in the columns the front sweep s0 first needs at
 //    step j (columns s0 .. p_0 + len_0 at j = 0, p_j .. p_j + len_j later, up
 //    to n at the front's last step) from HBM: the loads are issued at time
-//    j - 2, once the previous group's progress word is >= p_j + 2b - 1, and
-//    written to LDS at time j - 1.  The last column of a chunk, p_j + b - 1,
-//    is a step row of the previous group's last sweep, which writes it back
-//    down to its early-published beta only (offset b): the fill below is
-//    zeroed by that sweep's next step, so the I/O wave writes those zeros;
-//  * it writes back, at time t, the columns the group's last
-//    sweep finished at t - 1 (all columns up to its step row P; column P
-//    down to offset b) and publishes that sweep's next row P + len at t + 1,
-//    after its stores have completed (vmcnt(0)); the last write-back covers
-//    every remaining column and publishes INT_MAX.
+//    j - 3, once the previous group's progress word is >= p_j + 2b - 1, into
+//    one of two register buffers, and written to LDS at time j - 1.  The
+//    last column of a chunk, p_j + b - 1, is a step row of the previous
+//    group's last sweep, which writes it back down to its early-published
+//    beta only (offset b): the fill below is zeroed by that sweep's next
+//    step, so the I/O wave writes those zeros;
+//  * it writes back, at time t, the columns the group's last sweep finished
+//    at t - 1 (all columns up to its step row P; column P down to offset b)
+//    and publishes that sweep's next row P + len at t + 1, once those stores
+//    have completed; the last write-back covers every remaining column and
+//    publishes INT_MAX.
+// Every step issues the same number of memory operations in the same order
+// (kGChunk stores, one poll of the predecessor's progress word, kGChunk
+// loads; unused lanes address past the end of the band, which the buffer
+// range check drops), so the wait before the publish is a fixed count - the
+// newest kGChunk loads stay in flight - and the loads have two steps to land.
 // Within a time step the waves' LDS accesses are disjoint and every slot is
 // rewritten only after its previous column was written back, provided
 // kGW >= (2D + 2) b - D + 1; a CPU model of this schedule (random
 // interleavings of groups, access-set and slot checks) reproduces the
 // sequential chase.  Groups are handed out by ticket in increasing order, so
 // a group's predecessor is always running; every wait is bounded.
-constexpr int kGW = 256;     // window columns
+constexpr int kGW = 256;  // window columns
 // doubles per window column (2b <= 64 offsets).  Element (row R, column C)
 // sits at C * kGLD + R - C, so a band row runs through the window with stride
 // kGLD - 1 = 63 doubles (odd: the 32 lanes of a row access hit 32 distinct
@@ -265,11 +271,19 @@ constexpr int kGLD = 64;
 // most columns the front brings in at one step: b + 2 at b = 32 (the first
 // chunk s0 .. n of a sweep whose first step is also its last)
 constexpr int kGChunk = 34;
+constexpr unsigned kOutOfRange = 0x80000000u;  // byte offset past every band (< 2 GiB)
 
 __device__ __attribute__((always_inline)) inline void grp_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 __device__ __attribute__((always_inline)) inline void wave_lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// s_waitcnt with only the vector-memory count: vmcnt(N), N <= 63
+template <int N>
+__device__ __attribute__((always_inline)) inline void wait_vm() {
+  static_assert(N >= 0 && N <= 63, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
 
 __device__ inline int sweep_nsteps(int n, int b, int s) {
   int p = s + 1, len = min(b, n - p), k = 1;
@@ -282,10 +296,54 @@ __device__ inline int sweep_nsteps(int n, int b, int s) {
   }
 }
 
-// reflector of the chase's convention from x (lane < lenx), wave-wide
-__device__ inline void grp_reflector(double x, int lane, int lenx, double &uu, double &beta) {
-  const double nrm = sqrt(wave_sum(x * x));
-  const double x0 = __shfl(x, 0, 64);
+// 64-bit DPP move of v (rows outside RM read 0)
+template <int CTRL, int RM>
+__device__ __attribute__((always_inline)) inline double dpp_f64(double v) {
+  const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, RM, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, RM, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// sum over the 64 lanes, returned in every lane: quad, half-row and row
+// reductions by DPP, then row 15 -> rows 1 and 3 and lane 31 -> rows 2 and 3
+// broadcasts; lane 63 holds the total
+__device__ __attribute__((always_inline)) inline double wave_total(double v) {
+  v += dpp_f64<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_f64<0x140, 0xf>(v);  // row_mirror
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31
+  const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// v of lane ^ 32.  v_permlane32_swap exchanges one half of its destination
+// with the other half of its source; with both operands v, one result holds
+// this lane's own value and the other the partner's, so xor-ing out the own
+// value leaves the partner's whichever half was exchanged.
+__device__ __attribute__((always_inline)) inline double half_swap(double v) {
+  const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+  const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const unsigned plo = rl[0] ^ rl[1] ^ lo, phi = rh[0] ^ rh[1] ^ hi;
+  return __builtin_bit_cast(double, ((unsigned long long)phi << 32) | plo);
+}
+
+// reflector of the chase's convention from x (lane < lenx, zero elsewhere),
+// wave-wide; uu is zero for lanes >= lenx
+__device__ __attribute__((always_inline)) inline void grp_reflector(double x, int lane, int lenx, double &uu,
+                                                                   double &beta) {
+  const double nrm = sqrt(wave_total(x * x));
+  const unsigned long long xb = __builtin_bit_cast(unsigned long long, x);
+  const double x0 = __builtin_bit_cast(double, ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(
+                                                    (int)(unsigned)(xb >> 32), 0)
+                                                << 32) |
+                                                   (unsigned)__builtin_amdgcn_readlane((int)(unsigned)xb, 0));
   uu = 0.0;
   if (nrm > 0.0) {
     const double inv = 1.0 / nrm;
@@ -297,13 +355,24 @@ __device__ inline void grp_reflector(double x, int lane, int lenx, double &uu, d
   beta = (nrm > 0.0) ? (x0 >= 0.0 ? -nrm : nrm) : x0;
 }
 
+// one chunk of the front's columns in flight: registers + where they go
+// (columns lo .. hi-1; hi == lo when the step loads nothing)
+struct GrpChunk {
+  double v[kGChunk];
+  int lo, hi, syn;
+};
+
 template <int D>
 __global__ __launch_bounds__((D + 1) * 64, 1) void chase_group_kernel(
     int n, int b, double *__restrict__ band_all, long ldb, unsigned band_bytes, float *__restrict__ U_all, long ldu,
     int *__restrict__ com_all, unsigned *__restrict__ ticket, unsigned *__restrict__ err, int count, long band_stride,
     long u_stride, long com_stride) {
-  __shared__ double win[kGW * kGLD];
-  __shared__ double uS[D][32], yS[D][32], qS[D][32];
+  // the window, then one column-sized slot that takes the chunk writes of
+  // unused registers (every loaded register is written somewhere, so no load
+  // is dropped on some paths and the memory-operation count per step stays
+  // fixed)
+  __shared__ double win[kGW * kGLD + 64];
+  __shared__ double uS[D][32], yS[D][32];
   __shared__ int s_ticket, s_abort;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nsweeps = n - 2;
@@ -329,18 +398,26 @@ __global__ __launch_bounds__((D + 1) * 64, 1) void chase_group_kernel(
     const int tail_steps = sweep_nsteps(n, b, s0 + nsw - 1);
     const int front_steps = sweep_nsteps(n, b, s0);
     const int te = tail_steps - 1 + 2 * (nsw - 1);  // time step of the last sweep's last step
+    // the time loop runs t = -3 .. t_last, an even number of steps (the I/O
+    // wave's loop body is two steps, one per register buffer, with no exit
+    // between them: an exit there costs the loads their overlap)
+    const int t_last = te + 2 + ((te + 6) & 1);
 
     // Each role runs its own copy of the time loop (one barrier per time
     // step in every copy), so the roles' registers do not add up.
     if (wave < D) {
-      // ---- worker: sweep ws of this group
+      // ---- worker: sweep ws of this group.  Lane (c, h) = (lane & 31,
+      // lane >> 5) holds rows 2r + h of B1 column c; lane (ri, kc) = the same
+      // pair holds row ri, columns kc + 2r of the S and N blocks.  LDS reads
+      // are gathered into registers before any write of the same phase.
       const int ws = s0 + wave;
       const int w_steps = (wave < nsw) ? sweep_nsteps(n, b, ws) : 0;
+      const int c = lane & 31, h = lane >> 5;
       int wp = ws + 1, wlen = min(b, n - wp), cB1 = ws, colB1 = 1;
       bool wfirst = true;
       double wbeta = 0.0;
-      double *u = uS[wave], *yv = yS[wave], *qv = qS[wave];
-      for (int t = -2; t <= te + 2; ++t) {
+      double *u = uS[wave], *yv = yS[wave];
+      for (int t = -3; t <= t_last; ++t) {
         const int js = t - 2 * wave;
         if (js >= 0 && js < w_steps) {
           const int p = wp, len = wlen;
@@ -354,78 +431,80 @@ __global__ __launch_bounds__((D + 1) * 64, 1) void chase_group_kernel(
             if (lane < 32) u[lane] = uu;
             if (lane < len) U[(p + lane) + (long)ws * ldu] = (float)uu;
             wbeta = beta;
-            wave_lds_order();
           }
+          double u16[16];
+#pragma unroll
+          for (int r = 0; r < 16; r++) u16[r] = u[2 * r + h];
+          const double uc = u[c];
           // ---- B1 <- H B1 (columns 1 .. colB1-1); column 0 becomes beta e1
           {
-            const int c = lane & 31, h = lane >> 5;
             const bool act = (c >= 1 && c < colB1);
             // row p of column cB1 + c (lanes past the block read slot 0, unused)
             const int base = (c < colB1) ? ((cB1 + c) & (kGW - 1)) * kGLD + (p - cB1 - c) : 0;
             double bv[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) bv[r] = win[base + 2 * r + h];
             double tsum = 0.0;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
               const int i = 2 * r + h;
-              bv[r] = (act && i < len) ? win[base + i] : 0.0;
-              tsum += u[i] * bv[r];
+              if (!(act && i < len)) bv[r] = 0.0;
+              tsum += u16[r] * bv[r];
             }
-            tsum += __shfl_xor(tsum, 32, 64);
+            tsum += half_swap(tsum);
 #pragma unroll
             for (int r = 0; r < 16; r++) {
               const int i = 2 * r + h;
-              if (act && i < len) win[base + i] = bv[r] - u[i] * tsum;
-            }
-            if (c == 0) {
-#pragma unroll
-              for (int r = 0; r < 16; r++) {
-                const int i = 2 * r + h;
-                if (i < len && !(i == 0 && !wfirst)) win[base + i] = (i == 0) ? wbeta : 0.0;
-              }
+              if (act && i < len) win[base + i] = bv[r] - u16[r] * tsum;
+              if (c == 0 && i < len && !(i == 0 && !wfirst)) win[base + i] = (i == 0) ? wbeta : 0.0;
             }
           }
-          // ---- S <- H S H, N <- N H (S from the lower band, row ri, columns kk)
-          const int ri = lane & 31, kc = lane >> 5;
+          // ---- S <- H S H, N <- N H (S from the lower band)
+          const int ri = c, kc = h;
           double Sr[16], Nr[16];
 #pragma unroll
           for (int r = 0; r < 16; r++) {
             const int kk = kc + 2 * r;
             const int lo = min(ri, kk), hi = max(ri, kk);
-            Sr[r] = (ri < len && kk < len) ? win[((p + lo) & (kGW - 1)) * kGLD + (hi - lo)] : 0.0;
-            Nr[r] = (ri < len2 && kk < len) ? win[((p + kk) & (kGW - 1)) * kGLD + (len + ri - kk)] : 0.0;
+            Sr[r] = win[((p + lo) & (kGW - 1)) * kGLD + (hi - lo)];
+            Nr[r] = win[((p + kk) & (kGW - 1)) * kGLD + ((len + ri - kk) & 63)];
           }
           double yp = 0.0, qp = 0.0;
 #pragma unroll
           for (int r = 0; r < 16; r++) {
-            const double uk = u[kc + 2 * r];
-            yp += Sr[r] * uk;
-            qp += Nr[r] * uk;
+            const int kk = kc + 2 * r;
+            if (!(ri < len && kk < len)) Sr[r] = 0.0;
+            if (!(ri < len2 && kk < len)) Nr[r] = 0.0;
+            yp += Sr[r] * u16[r];
+            qp += Nr[r] * u16[r];
           }
-          yp += __shfl_xor(yp, 32, 64);
-          qp += __shfl_xor(qp, 32, 64);
-          if (kc == 0) {
-            yv[ri] = yp;
-            qv[ri] = qp;
+          yp += half_swap(yp);  // y = S u, row ri (both halves)
+          qp += half_swap(qp);  // q = N u, row ri
+          if (kc == 0) yv[ri] = yp;
+          double yk[16];
+#pragma unroll
+          for (int r = 0; r < 16; r++) yk[r] = yv[kc + 2 * r];
+          // u^T y over the 64 lanes counts every row twice
+          const double hpsi = 0.5 * (0.5 * wave_total(uc * yp));
+          const double wi = yp - hpsi * uc;
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const double wk = yk[r] - hpsi * u16[r];
+            Sr[r] -= uc * wk + wi * u16[r];
+            Nr[r] -= qp * u16[r];
           }
-          wave_lds_order();
-          const double psi = wave_sum((lane < 32) ? u[lane] * yv[lane] : 0.0);
-          const double ui = u[ri], wi = yv[ri] - 0.5 * psi * ui, qi = qv[ri];
 #pragma unroll
           for (int r = 0; r < 16; r++) {
             const int kk = kc + 2 * r;
-            const double uk = u[kk], wk = yv[kk] - 0.5 * psi * uk;
-            Sr[r] -= ui * wk + wi * uk;
-            Nr[r] -= qi * uk;
             if (ri >= kk && ri < len && kk < len) win[((p + kk) & (kGW - 1)) * kGLD + (ri - kk)] = Sr[r];
             if (ri < len2 && kk < len) win[((p + kk) & (kGW - 1)) * kGLD + (len + ri - kk)] = Nr[r];
           }
-          wave_lds_order();
-          // ---- next reflector from N[:, 0] (column p); its beta goes in now
+          // ---- next reflector from N[:, 0] (column p, held by lanes kc = 0);
+          // its beta goes in now
           if (!last) {
-            const double x = (lane < len2) ? win[(p & (kGW - 1)) * kGLD + len + lane] : 0.0;
+            const double x = (kc == 0 && ri < len2) ? Nr[0] : 0.0;
             double uu, beta;
             grp_reflector(x, lane, len2, uu, beta);
-            wave_lds_order();
             if (lane < 32) u[lane] = uu;
             if (lane < len2) U[(p + len + lane) + (long)ws * ldu] = (float)uu;
             if (lane == 0) win[(p & (kGW - 1)) * kGLD + len] = beta;
@@ -438,88 +517,119 @@ __global__ __launch_bounds__((D + 1) * 64, 1) void chase_group_kernel(
           }
         }
         grp_barrier();
-        if (s_abort) break;
+        if (((t + 3) & 1) && s_abort) break;
       }
     } else {
-      // ---- I/O wave.  Per time step: wait for last step's memory operations,
-      // publish last step's write-back, put chunk t+1 (loaded last step) into
-      // LDS, write back the columns the last sweep finished at t - 1, then
-      // issue the loads of chunk t+2 and a poll of the predecessor's progress.
-      int lj = 0, fp = s0 + 1, flen = min(b, n - s0 - 1);
+      // ---- I/O wave (see the schedule above); chunk j lives in buffer j & 1
+      int fp = s0 + 1, flen = min(b, n - s0 - 1), lj = 0;  // next front step to load
       int known = (g == 0) ? INT_MAX : -1, pv = -1;
-      bool pv_valid = false, pre_valid = false;
-      int pre_lo = 0, pre_hi = 0, pre_syn = -1;
-      double pre[kGChunk];
+      const int *pcom = com + (long)max(g - 1, 0) * kComStride;
       int tp = s0 + nsw, tlen = min(b, n - s0 - nsw), wb = s0, pval = 0;
       bool pend = false;
-      for (int t = -2; t <= te + 2; ++t) {
-        wait_mem();
+      GrpChunk ca, cb;
+#pragma unroll
+      for (int k = 0; k < kGChunk; k++) ca.v[k] = cb.v[k] = 0.0;
+      ca.lo = ca.hi = cb.lo = cb.hi = 0;
+      ca.syn = cb.syn = -1;
+      auto io_step = [&](int t, GrpChunk &ch) {
+        // all but the newest kGChunk loads (the other buffer's chunk) are done:
+        // this buffer's chunk, last step's stores and poll
+        wait_vm<kGChunk>();
         if (pend) {
+          if (pval == INT_MAX) wait_mem();
           if (lane == 0) st_dev(&com[(long)g * kComStride], pval);
           pend = false;
         }
-        if (pre_valid) {
+        known = max(known, pv);  // group 0: known is INT_MAX already
+        // chunk t + 1 -> LDS
 #pragma unroll
-          for (int k = 0; k < kGChunk; k++) {
-            const int c = pre_lo + k;
-            if (c < pre_hi && lane < b2) win[(c & (kGW - 1)) * kGLD + lane] = (c == pre_syn && lane > b) ? 0.0 : pre[k];
-          }
-          pre_valid = false;
+        for (int k = 0; k < kGChunk; k++) {
+          const int cc = ch.lo + k;
+          const int at = (cc < ch.hi && lane < b2) ? (cc & (kGW - 1)) * kGLD + lane : kGW * kGLD + lane;
+          win[at] = (cc == ch.syn && lane > b) ? 0.0 : ch.v[k];
         }
+        // write-back of the columns the last sweep finished at t - 1
         const int k = t - 1 - 2 * (nsw - 1);
-        if (k >= 0 && k < tail_steps) {
-          const int P = tp, len = tlen;
-          const bool last = (k == tail_steps - 1);
-          const int hi = last ? n : P + 1;
-          for (int c = wb; c < hi; c++) {
-            const int dmax = (last || c < P) ? b2 : b + 1;
-            if (lane < dmax) st_band(rb, (unsigned)lane + (unsigned)c * ld, win[(c & (kGW - 1)) * kGLD + lane]);
-          }
-          wb = hi;
-          pend = true;
-          pval = last ? INT_MAX : P + len;
-          tlen = max(0, min(b, n - P - len));
-          tp = P + len;
-        }
-        if (lj < front_steps) {
-          if (pv_valid) known = max(known, pv);
-          const int need = fp + 2 * b - 1;
-          if (known < need) {
-            const unsigned long long t0 = clock100();
-            while (true) {
-              known = max(known, ld_dev(&com[(long)(g - 1) * kComStride]));
-              if (known >= need) break;
-              if (clock100() - t0 > kWatchdogTicks) {
-                if (lane == 0) {
-                  atomicOr(err, (unsigned)EVD_E_TIMEOUT);
-                  s_abort = 1;
-                }
-                break;
-              }
-              __builtin_amdgcn_s_sleep(1);
+        const bool wbk = (k >= 0 && k < tail_steps);
+        const bool wlast = wbk && (k == tail_steps - 1);
+        const int P = tp;
+        const int whi = wbk ? (wlast ? n : P + 1) : wb;
+        int c0 = wb;
+        do {
+          // kGChunk stores (two halves of reads then stores); lanes past the
+          // columns or past a column's valid offsets store out of range
+#pragma unroll
+          for (int half = 0; half < 2; half++) {
+            double sv[kGChunk / 2];
+#pragma unroll
+            for (int k2 = 0; k2 < kGChunk / 2; k2++) {
+              const int cc = min(c0 + half * (kGChunk / 2) + k2, n - 1);
+              sv[k2] = win[(cc & (kGW - 1)) * kGLD + lane];
+            }
+#pragma unroll
+            for (int k2 = 0; k2 < kGChunk / 2; k2++) {
+              const int cc = c0 + half * (kGChunk / 2) + k2;
+              const int dmax = (wlast || cc < P) ? b2 : b + 1;
+              const unsigned off = (cc < whi && lane < dmax) ? ((unsigned)lane + (unsigned)cc * ld) * 8u : kOutOfRange;
+              __builtin_amdgcn_raw_buffer_store_b64(
+                  __builtin_bit_cast(__attribute__((__vector_size__(2 * sizeof(unsigned)))) unsigned, sv[k2]), rb, off,
+                  0, kSc1);
             }
           }
-          pre_lo = (lj == 0) ? s0 : fp;
-          pre_hi = (lj == front_steps - 1) ? n : fp + flen;
-          pre_syn = fp + b - 1;
-#pragma unroll
-          for (int k2 = 0; k2 < kGChunk; k2++) {
-            const int c = pre_lo + k2;
-            const unsigned off = (c < pre_hi && lane < b2) ? ((unsigned)lane + (unsigned)c * ld) * 8u : 0x80000000u;
-            pre[k2] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rb, off, 0, kSc1));
+          c0 += kGChunk;
+        } while (c0 < whi);
+        if (wbk) {
+          wb = whi;
+          pend = true;
+          pval = wlast ? INT_MAX : P + tlen;
+          const int nl = max(0, min(b, n - P - tlen));
+          tp = P + tlen;
+          tlen = nl;
+        }
+        // poll of the predecessor's progress word (its own word for group 0)
+        pv = ld_dev(pcom);
+        // loads of front step t + 3
+        const bool ld_now = (lj < front_steps);
+        if (ld_now && known < fp + 2 * b - 1 && !s_abort) {
+          const int need = fp + 2 * b - 1;
+          const unsigned long long t0 = clock100();
+          while (true) {
+            known = max(known, ld_dev(pcom));
+            if (known >= need) break;
+            if (clock100() - t0 > kWatchdogTicks) {
+              if (lane == 0) {
+                atomicOr(err, (unsigned)EVD_E_TIMEOUT);
+                s_abort = 1;
+              }
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
           }
-          pre_valid = true;
+        }
+        const int lo = (lj == 0) ? s0 : fp;
+        const int hi = ld_now ? ((lj == front_steps - 1) ? n : fp + flen) : lo;
+#pragma unroll
+        for (int k2 = 0; k2 < kGChunk; k2++) {
+          const int cc = lo + k2;
+          const unsigned off = (cc < hi && lane < b2) ? ((unsigned)lane + (unsigned)cc * ld) * 8u : kOutOfRange;
+          ch.v[k2] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rb, off, 0, kSc1));
+        }
+        ch.lo = lo;
+        ch.hi = hi;
+        ch.syn = fp + b - 1;
+        if (ld_now) {
           fp += flen;
           flen = min(b, n - fp);
           lj++;
         }
-        if (g > 0) {
-          pv = ld_dev(&com[(long)(g - 1) * kComStride]);
-          pv_valid = true;
-        }
         grp_barrier();
+      };
+      for (int t = -3; t <= t_last; t += 2) {
+        io_step(t, ca);
+        io_step(t + 1, cb);
         if (s_abort) break;
       }
+      wait_mem();
     }
     if (s_abort) {
       if (wave == D && lane == 0) st_dev(&com[(long)g * kComStride], INT_MAX);
@@ -636,15 +746,10 @@ int sb2st_batched(evd_ctx *ctx, int count, long n, long b, double *band, long ld
     unsigned *ticket = reinterpret_cast<unsigned *>(com + (size_t)count * com_stride);
     EVD_TRY(fill_u32(ctx, reinterpret_cast<unsigned *>(com), words, 0u));
     // sweeps per workgroup: the knob (0 = 1, one sweep per workgroup), at most
-    // what the LDS window holds
-    const int depth = std::min(ctx->tune.chase_depth > 0 ? ctx->tune.chase_depth : 1, group_depth_max(b));
-    if (b <= 32 && depth >= 6)
-      EVD_TRY(launch_chase_group<6>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride,
-                                    ticket));
-    else if (b <= 32 && depth >= 4)
-      EVD_TRY(launch_chase_group<4>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride,
-                                    ticket));
-    else if (b <= 32 && depth == 3)
+    // what the LDS window holds and at most 3 (with more than four waves a
+    // workgroup's waves share SIMDs and the grouped kernel's registers spill)
+    const int depth = std::min({ctx->tune.chase_depth > 0 ? ctx->tune.chase_depth : 1, group_depth_max(b), 3});
+    if (b <= 32 && depth == 3)
       EVD_TRY(launch_chase_group<3>(ctx, count, n, b, band, ldband, band_stride, U, ldu, u_stride, com, com_stride,
                                     ticket));
     else if (b <= 32 && depth == 2)

@@ -1,12 +1,13 @@
 """CPU model of the grouped bulge chase (chase.hip, chase_group_kernel): a
 workgroup owns D consecutive sweeps, one wave per sweep, stepping in
 lockstep (sweep s0 + i does step t - 2i at time step t) on a window of band
-columns held in LDS (column c in slot c mod W).  A load wave prefetches the
-front sweep's new columns two steps ahead from HBM once the previous group
-has published enough progress (synthesising the known zeros below the
-early-published beta of the last column); a store wave writes the columns the
-group's last sweep has finished back to HBM and publishes the progress one
-step later, after the stores have completed.
+columns held in LDS (column c in slot c mod W).  The I/O wave loads the front
+sweep's new columns from HBM three steps ahead (into registers) once the
+previous group has published enough progress, and writes them into the
+window one step ahead (synthesising the known zeros below the
+early-published beta of the last column); it writes the columns the group's
+last sweep has finished back to HBM and publishes the progress one step
+later, after the stores have completed.
 
 The model checks, for random interleavings of the groups' time steps:
   * the result equals the sequential chase (oracle) - D, E and the reflectors;
@@ -70,7 +71,7 @@ class Group:
             # the kernel's fixed chunk buffer holds b + 2 columns
             assert self.cols[-1][1] - self.cols[-1][0] <= b + 2
         self.need = [p + 2 * b - 1 for (p, _, _) in F]
-        self.t = -2
+        self.t = -3
         self.known = -1
         self.regs = {}  # step j -> {column: data} loaded, not yet in LDS
         self.win = {}  # slot -> [column, data(2b), retired_at]
@@ -102,7 +103,7 @@ class Group:
 
     # ---- one time step of the whole workgroup (all waves, then the barrier)
     def runnable(self):
-        j = self.t + 2  # load wave issues the loads of front step t + 2
+        j = self.t + 3  # the I/O wave issues the loads of front step t + 3
         if self.g == 0 or j >= len(self.cols):
             return True
         return max(self.known, self.m.com[self.g - 1]) >= self.need[j]
@@ -136,8 +137,8 @@ class Group:
                 self.win[self._slot(c)][2] = t  # retired at this step
             self.wb = hi
             self.pending = INF if last else P + ln
-        # -- load wave: LDS <- columns of front step t + 1 (loaded at t - 1),
-        # then issue the loads of front step t + 2
+        # -- I/O wave: LDS <- columns of front step t + 1 (loaded at t - 2),
+        # then issue the loads of front step t + 3
         j = t + 1
         if 0 <= j < len(self.cols) and j in self.regs:
             for c, data in self.regs.pop(j).items():
@@ -147,7 +148,7 @@ class Group:
                 self.win[self._slot(c)] = [c, data, None]
                 self.acc.setdefault("load", [set(), set()])[1].update((c + d, c) for d in range(2 * b))
             self.loaded_hi = self.cols[j][1]
-        j = t + 2
+        j = t + 3
         if 0 <= j < len(self.cols):
             if self.g > 0:
                 self.known = max(self.known, m.com[self.g - 1])
