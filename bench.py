@@ -198,6 +198,19 @@ def serving_bench(E, torch, nreq: int = 400, seed: int = 1, budget: int = 200 <<
     poi1 = run(1, arr, False, policy=1)
     poi_fp16 = run(slots, arr / 2.0, True, E.PREC_FP16)
     poi_plain = run(slots, arr / 2.0, True)
+    # the engine's throughput at the one-at-a-time p99: the same arrival
+    # sequence compressed to 1.0x and 1.4x the one-at-a-time rate; the
+    # highest of these rates (with the 2x run) whose p99 is no worse
+    sweep = {2.0: poi_plain}
+    for f in (1.0, 1.4):
+        sweep[f] = run(slots, arr / f, True)
+        print(f"[bench] serving engine at {f}x rate: {sweep[f]}", file=sys.stderr, flush=True)
+    ok = [f for f in sorted(sweep) if sweep[f]["p99_ms"] <= poi1["p99_ms"]]
+    matched = None
+    if ok:
+        f = ok[-1]
+        matched = {"rate_factor": f, "rate_per_s": f * rate} | sweep[f] | {
+            "throughput_ratio": sweep[f]["solves_per_s"] / poi1["solves_per_s"]}
     gctx.close()
     return {"requests": nreq, "mix": f"exact counts {counts}, 70% jobz=V, Gaussian entries, seed {seed}, "
                                       f"HBM budget {budget >> 30} GiB (one pool, lent per request)",
@@ -211,7 +224,9 @@ def serving_bench(E, torch, nreq: int = 400, seed: int = 1, budget: int = 200 <<
                         "throughput_ratio_fp16": poi_fp16["solves_per_s"] / poi1["solves_per_s"],
                         "p99_ratio_fp16": poi_fp16["p99_ms"] / poi1["p99_ms"],
                         "throughput_ratio_plain": poi_plain["solves_per_s"] / poi1["solves_per_s"],
-                        "p99_ratio_plain": poi_plain["p99_ms"] / poi1["p99_ms"]}}
+                        "p99_ratio_plain": poi_plain["p99_ms"] / poi1["p99_ms"],
+                        f"engine_{slots}slots_plain_rate_sweep": {f"{f}x": v for f, v in sorted(sweep.items())},
+                        "engine_plain_at_matched_p99": matched}}
 
 
 def captured_bench(E, torch, reps: int = 20):

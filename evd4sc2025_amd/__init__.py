@@ -556,20 +556,24 @@ def syevd_batched(ctx: Context, mats, b: int = 32, nb: int = 0, jobz: str = "V")
 
 
 # ---- autotuning (evd_autotune / evd_tune_cache_* in the C ABI)
-KNOBS = ("nb", "chase_groups", "bc_valu", "sbr_back_block")
+KNOBS = ("nb", "chase_groups", "bc_valu", "sbr_back_block", "chase_depth", "panel_rows", "gemm_tile", "bc_sweeps",
+         "bc_waves")
 
 
 def autotune(ctx: Context, n: int, b: int = 32, reps: int = 1) -> dict:
     """Tune the stage knobs for order-n problems at bandwidth b and the
     context's precision; the result is stored in the process-wide cache and
     returned as {knob: value}."""
-    out = (C.c_int * 4)()
+    out = (C.c_int * len(KNOBS))()
     ctx.call("evd_autotune", n, b, reps, out)
     return dict(zip(KNOBS, list(out)))
 
 
 def tune_cache_put(n: int, b: int, precision: int, knobs: dict):
-    arr = (C.c_int * 4)(*[int(knobs.get(k, 0)) for k in KNOBS])
+    unknown = set(knobs) - set(KNOBS)
+    if unknown:
+        raise ValueError(f"unknown tuning knobs {sorted(unknown)}")
+    arr = (C.c_int * len(KNOBS))(*[int(knobs.get(k, 0)) for k in KNOBS])
     rc = lib().evd_tune_cache_put(n, b, precision, arr)
     if rc:
         raise EvdError("evd_tune_cache_put", rc, "invalid entry")
@@ -577,7 +581,7 @@ def tune_cache_put(n: int, b: int, precision: int, knobs: dict):
 
 def tune_cache_get(n: int, b: int, precision: int):
     """{knob: value} of the cache entry covering (n, b, precision), or None."""
-    arr = (C.c_int * 4)()
+    arr = (C.c_int * len(KNOBS))()
     rc = lib().evd_tune_cache_get(n, b, precision, arr)
     if rc == 1:
         return None

@@ -15,6 +15,8 @@
 
 #include "../../include/evd4sc.h"
 
+typedef struct _rocblas_handle *rocblas_handle;
+
 namespace evd {
 
 constexpr int kWave = 64;
@@ -27,6 +29,9 @@ struct Tuning {
   int bc_valu = 0;         // 1: BC back transformation on the FP32 VALU kernel only
   int skinny_split = 0;    // split-K target (workgroups per CU) of long-K skinny products (0 = 4)
   int chase_depth = 0;     // sweeps per workgroup of the bulge chase (0 = 1: one sweep per workgroup)
+  int panel_rows = 0;      // panel QR rows per workgroup for panels <= 32 wide (0 = 512; 256)
+  int gemm_tile = 0;       // output tile of the large FP32 products (0 = auto, 1 = 128 x 128, 2 = 64 x 64)
+  int own_gemm = 0;        // 1: large plain FP32 products on this library's MFMA kernels, not rocBLAS
 };
 
 // Merge plan of the tridiagonal divide and conquer for one order n (host
@@ -79,6 +84,12 @@ struct evd_ctx {
   // the input screen takes max |a_ij| over the leading amax_rows[q] rows and
   // columns only (mixed-size batches: the pad diagonal is left out)
   const long *amax_rows = nullptr;
+
+  // library GEMM (rocBLAS) for the large plain FP32 products: handle on the
+  // context stream, deterministic (no atomics), with its own workspace so a
+  // call never allocates (also under capture)
+  rocblas_handle blas = nullptr;
+  void *blas_ws = nullptr;
 
   bool timing = false;
   hipEvent_t ev[EVD_STAGE_COUNT + 1] = {};

@@ -3,6 +3,8 @@
 #include <mutex>
 #include <tuple>
 
+#include <rocblas/rocblas.h>
+
 #include "common.hpp"
 
 namespace evd {
@@ -185,6 +187,31 @@ int check_device_error(evd_ctx *ctx, const char *where) {
 
 using namespace evd;
 
+// rocBLAS handle of a context (the large plain FP32 products): bound to the
+// context stream, atomics off (deterministic results: a product of one shape
+// always runs the same kernel the same way), host-side scalars and a fixed
+// device workspace so that no call allocates.
+static constexpr size_t kBlasWorkspace = (size_t)32 << 20;
+static int blas_open(evd_ctx *ctx) {
+  if (rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+    ctx->blas = nullptr;
+    return -1;
+  }
+  if (hipMalloc(&ctx->blas_ws, kBlasWorkspace) != hipSuccess) return -1;
+  if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success ||
+      rocblas_set_atomics_mode(ctx->blas, rocblas_atomics_not_allowed) != rocblas_status_success ||
+      rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host) != rocblas_status_success ||
+      rocblas_set_workspace(ctx->blas, ctx->blas_ws, kBlasWorkspace) != rocblas_status_success)
+    return -1;
+  return 0;
+}
+static void blas_close(evd_ctx *ctx) {
+  if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+  ctx->blas = nullptr;
+  if (ctx->blas_ws) hipFree(ctx->blas_ws);
+  ctx->blas_ws = nullptr;
+}
+
 extern "C" {
 
 int evd_ctx_create(int device, void *stream, evd_ctx **out) {
@@ -238,6 +265,10 @@ int evd_ctx_create(int device, void *stream, evd_ctx **out) {
       delete ctx;
       return EVD_E_HIP;
     }
+  if (blas_open(ctx) != 0) {
+    evd_ctx_destroy(ctx);
+    return EVD_E_HIP;
+  }
   *out = ctx;
   return 0;
 }
@@ -258,6 +289,7 @@ int evd_ctx_destroy(evd_ctx *ctx) {
     if (e) hipEventDestroy(e);
   for (auto &e : ctx->cev)
     if (e) hipEventDestroy(e);
+  blas_close(ctx);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -271,6 +303,7 @@ int evd_ctx_set_stream(evd_ctx *ctx, void *stream) {
     ctx->own_stream = false;
   }
   ctx->stream = static_cast<hipStream_t>(stream);
+  if (ctx->blas) rocblas_set_stream(ctx->blas, ctx->stream);
   return 0;
 }
 
@@ -315,10 +348,12 @@ int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value) {
   } knobs[] = {{"bc_sweeps", &ctx->tune.bc_sweeps, 0, 64},          {"bc_waves", &ctx->tune.bc_waves, 0, 16},
                {"chase_groups", &ctx->tune.chase_groups, 0, 1 << 20}, {"sbr_back_block", &ctx->tune.sbr_back_block, 0, 1 << 16},
                {"bc_valu", &ctx->tune.bc_valu, 0, 1},           {"skinny_split", &ctx->tune.skinny_split, 0, 4},
-               {"chase_depth", &ctx->tune.chase_depth, 0, 6}};
+               {"chase_depth", &ctx->tune.chase_depth, 0, 6},   {"panel_rows", &ctx->tune.panel_rows, 0, 512},
+               {"gemm_tile", &ctx->tune.gemm_tile, 0, 2},       {"own_gemm", &ctx->tune.own_gemm, 0, 1}};
   for (auto &kb : knobs)
     if (!strcmp(kb.name, name)) {
       if (value < kb.lo || value > kb.hi) return -3;
+      if (kb.slot == &ctx->tune.panel_rows && value != 0 && value != 256 && value != 512) return -3;
       *kb.slot = value;
       return 0;
     }

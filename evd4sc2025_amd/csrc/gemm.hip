@@ -14,10 +14,12 @@
 //    divide-and-conquer merges and the FP64 metrics.
 //
 // Operand staging: global -> registers (prefetch of tile k+1) -> LDS (k-major,
-// the MFMA lane index running along the contiguous LDS dimension, 4-float
-// padding) -> MFMA.  Operands are passed to the MFMA as (B-fragment,
+// the MFMA lane index running along the contiguous LDS dimension, rows padded
+// by 16 elements) -> MFMA.  Operands are passed to the MFMA as (B-fragment,
 // A-fragment) so that the accumulator lane index runs down a column of the
 // column-major C, which makes the epilogue stores contiguous.
+#include <rocblas/rocblas.h>
+
 #include "common.hpp"
 
 namespace evd {
@@ -39,12 +41,58 @@ template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, int BK =
 struct TileLoader {
   static constexpr int NA = BM * BK / 256;
   static constexpr int NB = BN * BK / 256;
-  static constexpr int LDSA = BM + 4;
-  static constexpr int LDSB = BN + 4;
+  // LDS row pitch: an FP32 operand fragment reads 16 consecutive elements
+  // from each of 4 k-rows; a pitch of 16 (mod 64) dwords puts the 4 rows on
+  // disjoint banks.  FP64 keeps a 4-element pad (the FP64 GEMM measured
+  // 47.6 TFLOP/s with it at 8192^3, 44.5 with the 16-element pad).
+  static constexpr int LDSA = BM + (sizeof(T) == 4 ? 16 : 4);
+  static constexpr int LDSB = BN + (sizeof(T) == 4 ? 16 : 4);
   T ra[NA], rb[NB];
 
-  __device__ inline void load(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M, int N,
-                              int m0, int n0, int k0, int ke, int tid) {
+  // 16-byte operand loads.  An operand goes through them when its base is
+  // 16-byte aligned and its leading dimension a multiple of VW elements (the
+  // caller's check, va / vb); each thread then moves VW consecutive elements
+  // of the operand's contiguous dimension (M or N for A / B^T, K for A^T / B)
+  // per load.  A vector that runs past the matrix edge or the K range falls
+  // back to single elements.  ra / rb hold the elements in the vector
+  // mapping, which store(..., va, vb) undoes; the element values (hence the
+  // products) are those of the scalar path.
+  // (FP32 only: for FP64 the two-element vectors slowed the grouped D&C
+  // GEMM from 81 to 97 ms at n = 16384)
+  static constexpr int VW = 16 / sizeof(T);
+  static constexpr bool CAN_VA = sizeof(T) == 4 && (NA % VW == 0) && ((!TA ? BM : BK) % VW == 0);
+  static constexpr bool CAN_VB = sizeof(T) == 4 && (NB % VW == 0) && ((TB ? BN : BK) % VW == 0);
+  typedef T vecT __attribute__((ext_vector_type(16 / sizeof(T))));
+
+  __device__ inline void load_a(const T *__restrict__ A, long lda, int M, int m0, int k0, int ke, int tid, bool va) {
+    if (CAN_VA && va) {
+#pragma unroll
+      for (int v = 0; v < NA / VW; v++) {
+        const int vidx = tid + 256 * v;
+        int i, kk;
+        if (!TA) {  // VW consecutive rows of one column
+          i = (vidx % (BM / VW)) * VW;
+          kk = vidx / (BM / VW);
+        } else {  // VW consecutive k of one row of op(A)
+          kk = (vidx % (BK / VW)) * VW;
+          i = vidx / (BK / VW);
+        }
+        const int gi = m0 + i, gk = k0 + kk;
+        const bool full = !TA ? (gi + VW <= M && gk < ke) : (gi < M && gk + VW <= ke);
+        if (full) {
+          const vecT x = *reinterpret_cast<const vecT *>(TA ? &A[gk + (long)gi * lda] : &A[gi + (long)gk * lda]);
+#pragma unroll
+          for (int e = 0; e < VW; e++) ra[v * VW + e] = x[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < VW; e++) {
+            const int ri = !TA ? gi + e : gi, rk = !TA ? gk : gk + e;
+            ra[v * VW + e] = (ri < M && rk < ke) ? (TA ? A[rk + (long)ri * lda] : A[ri + (long)rk * lda]) : T(0);
+          }
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < NA; q++) {
       int idx = tid + 256 * q;
@@ -60,6 +108,37 @@ struct TileLoader {
       T v = T(0);
       if (gi < M && gk < ke) v = TA ? A[gk + (long)gi * lda] : A[gi + (long)gk * lda];
       ra[q] = v;
+    }
+  }
+
+  __device__ inline void load_b(const T *__restrict__ B, long ldb, int N, int n0, int k0, int ke, int tid, bool vb) {
+    if (CAN_VB && vb) {
+#pragma unroll
+      for (int v = 0; v < NB / VW; v++) {
+        const int vidx = tid + 256 * v;
+        int j, kk;
+        if (TB) {  // VW consecutive columns j of op(B) = rows of B
+          j = (vidx % (BN / VW)) * VW;
+          kk = vidx / (BN / VW);
+        } else {  // VW consecutive k of one column
+          kk = (vidx % (BK / VW)) * VW;
+          j = vidx / (BK / VW);
+        }
+        const int gj = n0 + j, gk = k0 + kk;
+        const bool full = TB ? (gj + VW <= N && gk < ke) : (gj < N && gk + VW <= ke);
+        if (full) {
+          const vecT x = *reinterpret_cast<const vecT *>(TB ? &B[gj + (long)gk * ldb] : &B[gk + (long)gj * ldb]);
+#pragma unroll
+          for (int e = 0; e < VW; e++) rb[v * VW + e] = x[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < VW; e++) {
+            const int rj = TB ? gj + e : gj, rk = TB ? gk : gk + e;
+            rb[v * VW + e] = (rj < N && rk < ke) ? (TB ? B[rj + (long)rk * ldb] : B[rk + (long)rj * ldb]) : T(0);
+          }
+        }
+      }
+      return;
     }
 #pragma unroll
     for (int q = 0; q < NB; q++) {
@@ -79,7 +158,25 @@ struct TileLoader {
     }
   }
 
-  __device__ inline void store(T *As, T *Bs, int tid) {
+  __device__ inline void store_a(T *As, int tid, bool va) {
+    if (CAN_VA && va) {
+#pragma unroll
+      for (int v = 0; v < NA / VW; v++) {
+        const int vidx = tid + 256 * v;
+        if (!TA) {
+          const int i = (vidx % (BM / VW)) * VW, kk = vidx / (BM / VW);
+          vecT x;
+#pragma unroll
+          for (int e = 0; e < VW; e++) x[e] = ra[v * VW + e];
+          *reinterpret_cast<vecT *>(&As[kk * LDSA + i]) = x;
+        } else {
+          const int kk = (vidx % (BK / VW)) * VW, i = vidx / (BK / VW);
+#pragma unroll
+          for (int e = 0; e < VW; e++) As[(kk + e) * LDSA + i] = ra[v * VW + e];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < NA; q++) {
       int idx = tid + 256 * q;
@@ -92,6 +189,27 @@ struct TileLoader {
         i = idx / BK;
       }
       As[kk * LDSA + i] = ra[q];
+    }
+  }
+
+  __device__ inline void store_b(T *Bs, int tid, bool vb) {
+    if (CAN_VB && vb) {
+#pragma unroll
+      for (int v = 0; v < NB / VW; v++) {
+        const int vidx = tid + 256 * v;
+        if (TB) {
+          const int j = (vidx % (BN / VW)) * VW, kk = vidx / (BN / VW);
+          vecT x;
+#pragma unroll
+          for (int e = 0; e < VW; e++) x[e] = rb[v * VW + e];
+          *reinterpret_cast<vecT *>(&Bs[kk * LDSB + j]) = x;
+        } else {
+          const int kk = (vidx % (BK / VW)) * VW, j = vidx / (BK / VW);
+#pragma unroll
+          for (int e = 0; e < VW; e++) Bs[(kk + e) * LDSB + j] = rb[v * VW + e];
+        }
+      }
+      return;
     }
 #pragma unroll
     for (int q = 0; q < NB; q++) {
@@ -106,6 +224,17 @@ struct TileLoader {
       }
       Bs[kk * LDSB + j] = rb[q];
     }
+  }
+
+  // element (scalar) mapping: the split-precision staging reads ra / rb in it
+  __device__ inline void load(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M, int N,
+                              int m0, int n0, int k0, int ke, int tid, bool va = false, bool vb = false) {
+    load_a(A, lda, M, m0, k0, ke, tid, va);
+    load_b(B, ldb, N, n0, k0, ke, tid, vb);
+  }
+  __device__ inline void store(T *As, T *Bs, int tid, bool va = false, bool vb = false) {
+    store_a(As, tid, va);
+    store_b(Bs, tid, vb);
   }
 };
 
@@ -133,15 +262,19 @@ __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   L ld;
+  // 16-byte loads where the operand's alignment allows them
+  constexpr int VW = L::VW;
+  const bool va = ((reinterpret_cast<unsigned long>(A) & 15) == 0) && (lda % VW == 0);
+  const bool vb = ((reinterpret_cast<unsigned long>(B) & 15) == 0) && (ldb % VW == 0);
   int nk = ke > kb ? (ke - kb + BK - 1) / BK : 0;
   if (nk > 0) {
-    ld.load(A, lda, B, ldb, M, N, m0, n0, kb, ke, tid);
-    ld.store(As0, Bs0, tid);
+    ld.load(A, lda, B, ldb, M, N, m0, n0, kb, ke, tid, va, vb);
+    ld.store(As0, Bs0, tid, va, vb);
   }
   __syncthreads();
   for (int t = 0; t < nk; t++) {
     const int cur = t & 1;
-    if (t + 1 < nk) ld.load(A, lda, B, ldb, M, N, m0, n0, kb + (t + 1) * BK, ke, tid);
+    if (t + 1 < nk) ld.load(A, lda, B, ldb, M, N, m0, n0, kb + (t + 1) * BK, ke, tid, va, vb);
     const T *As = As0 + cur * SA;
     const T *Bs = Bs0 + cur * SB;
 #pragma unroll
@@ -157,7 +290,7 @@ __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T 
 #pragma unroll
         for (int ni = 0; ni < NI; ni++) acc[mi][ni] = mfma16(b[ni], a[mi], acc[mi][ni]);
     }
-    if (t + 1 < nk) ld.store(As0 + (cur ^ 1) * SA, Bs0 + (cur ^ 1) * SB, tid);
+    if (t + 1 < nk) ld.store(As0 + (cur ^ 1) * SA, Bs0 + (cur ^ 1) * SB, tid, va, vb);
     __syncthreads();
   }
 }
@@ -169,8 +302,8 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
                                                     int ksplit, float *__restrict__ P, long batches) {
   using L = TileLoader<TA, TB, BM, BN, WM, WN, float, BK>;
   constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
-  __shared__ float As[2 * BK * L::LDSA];
-  __shared__ float Bs[2 * BK * L::LDSB];
+  __shared__ __align__(16) float As[2 * BK * L::LDSA];
+  __shared__ __align__(16) float Bs[2 * BK * L::LDSB];
   const long bz = blockIdx.z;
   const long batch = bz / ksplit;
   const int kz = (int)(bz % ksplit);
@@ -287,7 +420,9 @@ static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, con
   if (m <= 32 && n > 64)
     return launch_sgemm<TA, TB, 32, 256, 1, 4>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   long big_tiles = ((m + 127) / 128) * ((n + 127) / 128) * shape_batch(ctx, batch);
-  if (big_tiles >= target || (m > 64 && n > 64 && k < 1024))
+  // the gemm_tile knob forces 128 x 128 (1) or 64 x 64 (2) output tiles
+  if (ctx->tune.gemm_tile == 1 ||
+      (ctx->tune.gemm_tile == 0 && (big_tiles >= target || (m > 64 && n > 64 && k < 1024))))
     return launch_sgemm<TA, TB, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   return launch_sgemm<TA, TB, 64, 64, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128);
 }
@@ -306,6 +441,35 @@ static int sgemm_split(evd_ctx *ctx, bool TA, bool TB, long m, long n, long k, f
                        long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
                        bool *handled);
 
+// Large plain FP32 products (an output of >= 2^22 elements, i.e. >= 256
+// tiles of 128 x 128, and >= 2^31 flops per problem) go to rocBLAS on the
+// context's handle: its tuned kernels run these shapes at 126-154 TFLOP/s on
+// MI355X against about 100 for the tile kernels here.  Products with a
+// small output and a long K stay here: without atomics (needed for
+// reproducible bits) the library does not split K, and a 256 x 256 x 16384
+// product then ran on a handful of workgroups.  A batch is one call per
+// problem, so every problem's arithmetic (the library's kernel choice for
+// that shape) is that of a single product.  16-bit split products stay here.
+static bool library_gemm(const evd_ctx *ctx, long m, long n, long k) {
+  return ctx->blas && !ctx->tune.own_gemm && (double)m * n >= 4194304.0 && 2.0 * m * n * k >= 2147483648.0 &&
+         m < (1L << 31) && n < (1L << 31) && k < (1L << 31);
+}
+
+static int sgemm_library(evd_ctx *ctx, bool TA, bool TB, long m, long n, long k, float alpha, const float *A,
+                         long lda, long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC,
+                         long batch) {
+  const rocblas_operation oa = TA ? rocblas_operation_transpose : rocblas_operation_none;
+  const rocblas_operation ob = TB ? rocblas_operation_transpose : rocblas_operation_none;
+  for (long q = 0; q < batch; q++) {
+    const rocblas_status st = rocblas_sgemm(ctx->blas, oa, ob, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha,
+                                            A + q * sA, (rocblas_int)lda, B + q * sB, (rocblas_int)ldb, &beta,
+                                            C + q * sC, (rocblas_int)ldc);
+    if (st != rocblas_status_success)
+      return set_error(ctx, EVD_E_HIP, "rocblas_sgemm(%ld x %ld x %ld) failed: status %d", m, n, k, (int)st);
+  }
+  return 0;
+}
+
 int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, const float *A, long lda,
           const float *B, long ldb, float beta, float *C, long ldc, long batch, long sA, long sB, long sC) {
   if (m <= 0 || n <= 0 || batch <= 0) return 0;
@@ -318,6 +482,9 @@ int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, c
     return 0;
   }
   bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+  if (ctx->precision == EVD_PREC_FP32 && library_gemm(ctx, m, n, k) && lda < (1L << 31) && ldb < (1L << 31) &&
+      ldc < (1L << 31))
+    return sgemm_library(ctx, TA, TB, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   if (ctx->precision != EVD_PREC_FP32) {
     bool handled = false;
     const int r = sgemm_split(ctx, TA, TB, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, &handled);
@@ -340,8 +507,8 @@ __global__ __launch_bounds__(256) void syr2k_kernel(int n, int k, float alpha, c
   B += blockIdx.y * sB;
   C += blockIdx.y * sC;
   using L = TileLoader<false, true, BM, BN, WM, WN, float, BK>;
-  __shared__ float As[2 * BK * L::LDSA];
-  __shared__ float Bs[2 * BK * L::LDSB];
+  __shared__ __align__(16) float As[2 * BK * L::LDSA];
+  __shared__ __align__(16) float Bs[2 * BK * L::LDSB];
   const long t = blockIdx.x;
   long ti = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
@@ -707,14 +874,17 @@ static int syr2k_split(evd_ctx *ctx, long n, long k, float alpha, const float *A
 }
 
 // ----------------------------------------------------------------- FP64
-template <bool TA, bool TB>
+// 64 x 64 output tile on 4 waves (2 x 2, each 32 x 32 of 16 x 16 MFMA
+// tiles); a 128 x 128 tile (1 workgroup per CU by registers) measured
+// slower (41.9 against 47.6 TFLOP/s at 8192^3).
+template <bool TA, bool TB, int BM>
 __global__ __launch_bounds__(256) void dgemm_kernel(int M, int N, int K, double alpha, const double *__restrict__ A,
                                                     long lda, const double *__restrict__ B, long ldb, double beta,
                                                     double *__restrict__ C, long ldc) {
-  constexpr int BM = 64, BN = 64, WM = 2, WN = 2, MI = 2, NI = 2;
+  constexpr int BN = BM, WM = 2, WN = 2, MI = BM / WM / 16, NI = BN / WN / 16;
   using L = TileLoader<TA, TB, BM, BN, WM, WN, double>;
-  __shared__ double As[2 * kBK * L::LDSA];
-  __shared__ double Bs[2 * kBK * L::LDSB];
+  __shared__ __align__(16) double As[2 * kBK * L::LDSA];
+  __shared__ __align__(16) double Bs[2 * kBK * L::LDSB];
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   f64x4 acc[MI][NI];
 #pragma unroll
@@ -740,23 +910,29 @@ __global__ __launch_bounds__(256) void dgemm_kernel(int M, int N, int K, double 
   }
 }
 
+template <bool TA, bool TB>
+static void launch_dgemm(evd_ctx *ctx, long m, long n, long k, double alpha, const double *A, long lda,
+                         const double *B, long ldb, double beta, double *C, long ldc) {
+  dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64));
+  dgemm_kernel<TA, TB, 64><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
 int dgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, double alpha, const double *A, long lda,
           const double *B, long ldb, double beta, double *C, long ldc) {
   if (m <= 0 || n <= 0) return 0;
   bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
-  dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64));
   if (k <= 0) {
     // C = beta C through a zero-K product
     k = 0;
   }
   if (!TA && !TB)
-    dgemm_kernel<false, false><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+    launch_dgemm<false, false>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   else if (TA && !TB)
-    dgemm_kernel<true, false><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+    launch_dgemm<true, false>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   else if (!TA && TB)
-    dgemm_kernel<false, true><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+    launch_dgemm<false, true>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   else
-    dgemm_kernel<true, true><<<grid, 256, 0, ctx->stream>>>((int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+    launch_dgemm<true, true>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   EVD_LAUNCHED(ctx);
   return 0;
 }
@@ -766,8 +942,8 @@ __global__ __launch_bounds__(256) void dgemm_grouped_kernel(const DGemmProblem *
                                                             int total_tiles) {
   constexpr int BM = 64, BN = 64, WM = 2, WN = 2, MI = 2, NI = 2;
   using L = TileLoader<false, false, BM, BN, WM, WN, double>;
-  __shared__ double As[2 * kBK * L::LDSA];
-  __shared__ double Bs[2 * kBK * L::LDSB];
+  __shared__ __align__(16) double As[2 * kBK * L::LDSA];
+  __shared__ __align__(16) double Bs[2 * kBK * L::LDSB];
   const int t = blockIdx.x;
   if (t >= total_tiles) return;
   int lo = 0, hi = nprob - 1;

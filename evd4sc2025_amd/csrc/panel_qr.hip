@@ -425,6 +425,10 @@ template <typename T>
 int panel_qr_t(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *Y, long ldy, T *W, long ldw, T *R, long ldr,
                T *Aout, long ldo, double *tau_out) {
   if (m <= 0 || nc <= 0) return 0;
+  // rows per workgroup (the panel_rows knob): 512 (two per thread) by
+  // default; 256 doubles the workgroups that share the column exchanges
+  if (nc <= 32 && ctx->tune.panel_rows == 256)
+    return launch_panel<32, 1, T>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
   if (nc <= 32) return launch_panel<32, 2, T>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
   if (nc <= 64) return launch_panel<64, 1, T>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
   return set_error(ctx, EVD_E_UNSUPPORTED, "panel width %ld > 64", nc);

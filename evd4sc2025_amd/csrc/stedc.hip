@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <cfloat>
 
+#include <rocblas/rocblas.h>
+
 #include "common.hpp"
 
 namespace evd {
@@ -756,15 +758,25 @@ static int build_tree(std::vector<TreeNode> &nodes, int start, int size) {
   return id;
 }
 
-size_t stedc_workspace(long n) {
+// Merges of order >= kDcLibMinN form their FP64 products with rocBLAS (the
+// library's dgemm runs these shapes at 71-75 TFLOP/s on MI355X, the grouped
+// tile kernel here at about 48) when the host may read the level's
+// deflation counts, i.e. outside graph capture; the product goes through a
+// temporary of (n/2 + 1) x n doubles and is scattered to its columns.
+constexpr long kDcLibMinN = 4096;
+static size_t dc_lib_temp_words(long n) { return n >= kDcLibMinN ? (size_t)(n / 2 + 1) * n : 0; }
+
+static size_t dc_per_problem(long n) {
   size_t b = 0;
   b += align256((size_t)n * n * sizeof(double)) * 2;  // Qc, Uc
   b += align256((size_t)n * sizeof(double)) * 12;
   b += align256((size_t)n * sizeof(int)) * 12;
   b += align256((size_t)n * sizeof(MergeOut));
   b += align256((size_t)2 * n * sizeof(DGemmProblem));
-  return b + (1 << 16);
+  return b;
 }
+
+size_t stedc_workspace(long n) { return dc_per_problem(n) + align256(dc_lib_temp_words(n) * sizeof(double)) + (1 << 16); }
 
 // Merge plan of order n, built once per context: the tree (halving splits
 // down to single rows), merges grouped by height, the split flags and the
@@ -860,7 +872,49 @@ int stedc(evd_ctx *ctx, long n, double *D, double *E, double *Z, long ldz) {
 
 size_t stedc_batched_workspace(int count, long n) {
   const int per = (int)std::max<long>(1, 65535 / ((n + 1) / 2));
-  return stedc_workspace(n) * (size_t)std::min(count, per);
+  return dc_per_problem(n) * (size_t)std::min(count, per) + align256(dc_lib_temp_words(n) * sizeof(double)) +
+         (1 << 16);
+}
+
+// C[:, colmap[j]] = T[:, j] (j < n, rows < m) of one merge product
+__global__ void dc_scatter_cols_kernel(int m, int n, const double *__restrict__ T, double *__restrict__ C, long ldc,
+                                       const int *__restrict__ colmap) {
+  const long total = (long)m * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % m), j = (int)(e / m);
+    const int oc = colmap ? colmap[j] : j;
+    C[r + (long)oc * ldc] = T[e];
+  }
+}
+
+// One merge level's products on rocBLAS: the problem list the device built
+// (sizes from the deflation results) is read back, each product goes to the
+// temporary T and is scattered to its output columns.  Per problem, so the
+// problems of a batch get the arithmetic of their single solves.
+static int dc_level_library(evd_ctx *ctx, const DGemmProblem *dprob, int nprob, double *T, size_t t_words) {
+  std::vector<DGemmProblem> hp((size_t)nprob);
+  EVD_HIP(ctx, hipMemcpyAsync(hp.data(), dprob, sizeof(DGemmProblem) * nprob, hipMemcpyDeviceToHost, ctx->stream));
+  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const double one = 1.0, zero = 0.0;
+  for (const DGemmProblem &p : hp) {
+    if (p.m <= 0 || p.n <= 0) continue;
+    if ((size_t)p.m * p.n > t_words)
+      return set_error(ctx, EVD_E_UNSUPPORTED, "D&C product %d x %d exceeds its temporary", p.m, p.n);
+    if (p.k > 0) {
+      const rocblas_status st = rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, p.m, p.n,
+                                              p.k, &one, p.A, (rocblas_int)p.lda, p.B, (rocblas_int)p.ldb, &zero, T,
+                                              p.m);
+      if (st != rocblas_status_success)
+        return set_error(ctx, EVD_E_HIP, "rocblas_dgemm(%d x %d x %d) failed: status %d", p.m, p.n, p.k, (int)st);
+    } else {
+      EVD_HIP(ctx, hipMemsetAsync(T, 0, sizeof(double) * p.m * p.n, ctx->stream));
+    }
+    const long total = (long)p.m * p.n;
+    dc_scatter_cols_kernel<<<(unsigned)std::min<long>((total + 255) / 256, 8192), 256, 0, ctx->stream>>>(
+        p.m, p.n, T, p.C, p.ldc, p.colmap);
+    EVD_LAUNCHED(ctx);
+  }
+  return 0;
 }
 
 // `count` same-size tridiagonals (D, E of problem q at q n, Z at q zs): every
@@ -903,6 +957,12 @@ int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double 
   const int *dtiles = reinterpret_cast<const int *>(pdev + align256(sizeof(MergeDesc) * nmerge) +
                                                     align256(sizeof(int) * nt));
   MergeOut *dmo = static_cast<MergeOut *>(arena_alloc(ctx, sizeof(MergeOut) * nmerge));
+  // temporary of the library products (large merges, eager solves)
+  double *libT = nullptr;
+  if (ctx->blas && !ctx->tune.own_gemm && !ctx->capturing && dc_lib_temp_words(n) > 0) {
+    libT = arena_alloc_n<double>(ctx, dc_lib_temp_words(n));
+    if (!libT) return EVD_E_NOMEM;
+  }
   DGemmProblem *dprob = static_cast<DGemmProblem *>(arena_alloc(ctx, sizeof(DGemmProblem) * 2 * nmerge));
   double *nrm = arena_alloc_n<double>(ctx, (size_t)count);
   if (!dmo || !dprob || !nrm) return EVD_E_NOMEM;
@@ -949,7 +1009,10 @@ int stedc_batched(evd_ctx *ctx, int count, long n, double *D, double *E, double 
     dc_problems_kernel<<<(unsigned)((nm + 255) / 256), 256, 0, ctx->stream>>>(nm, lmd, lmo, dtiles + 2 * off, sc, ldq,
                                                                              Z, ldz, dprob);
     EVD_LAUNCHED(ctx);
-    EVD_TRY(dgemm_grouped(ctx, dprob, 2 * nm, plan->level_tiles[h]));
+    if (libT && maxN >= kDcLibMinN && !ctx->capturing)
+      EVD_TRY(dc_level_library(ctx, dprob, 2 * nm, libT, dc_lib_temp_words(n)));
+    else
+      EVD_TRY(dgemm_grouped(ctx, dprob, 2 * nm, plan->level_tiles[h]));
     const unsigned gd = (unsigned)std::min<long>(((long)maxN * maxN + 255) / 256, 1024);
     dc_copy_deflated_kernel<<<dim3(gd, nm), 256, 0, ctx->stream>>>(lmd, lmo, sc, ldq, Z, ldz);
     EVD_LAUNCHED(ctx);

@@ -435,6 +435,19 @@ extern "C" int evd_syevd_f32(evd_ctx *ctx, char jobz, long n, long b, long nb, f
   return syevd_call(ctx, jobz, n, b, nb, A, lda, w, Z, ldz);
 }
 
+// knobs the caller left at 0 from a tuning-cache entry (chase workgroups
+// scale with the problems of a batch)
+static void fill_knobs(Tuning *dst, const Tuning &t, int count) {
+  if (!dst->chase_groups) dst->chase_groups = t.chase_groups * std::max(1, count);
+  if (!dst->bc_valu) dst->bc_valu = t.bc_valu;
+  if (!dst->sbr_back_block) dst->sbr_back_block = t.sbr_back_block;
+  if (!dst->chase_depth) dst->chase_depth = t.chase_depth;
+  if (!dst->panel_rows) dst->panel_rows = t.panel_rows;
+  if (!dst->gemm_tile) dst->gemm_tile = t.gemm_tile;
+  if (!dst->bc_sweeps) dst->bc_sweeps = t.bc_sweeps;
+  if (!dst->bc_waves) dst->bc_waves = t.bc_waves;
+}
+
 // The arena bytes syevd_call / syevd_batched_call will ask for on this
 // context (the same knob and nb resolution as the calls).
 static void resolve_knobs(const evd_ctx *ctx, long n, long b, long *nb, long *sbr_knob) {
@@ -463,9 +476,7 @@ int evd::syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, 
   int tnb = 0;
   Tuning t;
   if (ctx->tune_cache && tune_lookup(n, b, ctx->precision, &tnb, &t)) {
-    if (!ctx->tune.chase_groups) ctx->tune.chase_groups = t.chase_groups;
-    if (!ctx->tune.bc_valu) ctx->tune.bc_valu = t.bc_valu;
-    if (!ctx->tune.sbr_back_block) ctx->tune.sbr_back_block = t.sbr_back_block;
+    fill_knobs(&ctx->tune, t, 1);
   }
   if (nb == 0) nb = tnb > 0 ? tnb : b * std::max<long>(1, 1024 / b);
   int s = arena_ensure(ctx, syevd_workspace(n, b, nb, jobz, ctx->precision, ctx->tune.sbr_back_block));
@@ -508,9 +519,7 @@ int evd::syevd_batched_call(evd_ctx *ctx, char jobz, int count, long n, long b, 
   int tnb = 0;
   Tuning t;
   if (ctx->tune_cache && tune_lookup(n, b, ctx->precision, &tnb, &t)) {
-    if (!ctx->tune.chase_groups) ctx->tune.chase_groups = t.chase_groups * std::max(1, count);
-    if (!ctx->tune.bc_valu) ctx->tune.bc_valu = t.bc_valu;
-    if (!ctx->tune.sbr_back_block) ctx->tune.sbr_back_block = t.sbr_back_block;
+    fill_knobs(&ctx->tune, t, count);
   }
   if (nb == 0) nb = tnb > 0 ? tnb : b * std::max<long>(1, 1024 / b);
   int s = arena_ensure(ctx, syevd_batched_workspace(count, n, b, nb, jobz, ctx->precision, ctx->tune.sbr_back_block));

@@ -11,6 +11,13 @@
 //                   (0) or on the FP32 VALU kernel (1)
 //   sbr_back_block  WY block width of the SBR back transformation (MFMA tile
 //                   of its GEMMs along k)
+//   chase_depth     sweeps per workgroup of the bulge chase (sweep depth)
+//   panel_rows      panel QR rows per workgroup (the panel's width in
+//                   workgroups: 512 or 256 rows each)
+//   gemm_tile       output tile of the large FP32 MFMA products (128 x 128 or
+//                   64 x 64)
+//   bc_sweeps       sweeps per block of the VALU BC back transformation
+//   bc_waves        pipelined waves (sweep blocks in flight) of that kernel
 //
 // evd_autotune runs the eigensolver on a synthetic matrix of the requested
 // order for candidate values, one knob at a time (coordinate descent, each
@@ -19,6 +26,8 @@
 // knob the caller left at 0 from that entry; the cache is process-wide and
 // saved / loaded as text, one entry per line:
 //   <n_bucket> <b> <precision> <nb> <chase_groups> <bc_valu> <sbr_back_block>
+//   <chase_depth> <panel_rows> <gemm_tile> <bc_sweeps> <bc_waves>
+// (lines with only the first seven fields load with the others at 0).
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -31,8 +40,33 @@ namespace {
 
 typedef std::tuple<long, long, int> Key;
 
+// the knob vector of the C ABI, in its order (EVD_TUNE_KNOBS entries)
 struct Entry {
-  int nb = 0, chase_groups = 0, bc_valu = 0, sbr_back_block = 0;
+  int nb = 0, chase_groups = 0, bc_valu = 0, sbr_back_block = 0, chase_depth = 0, panel_rows = 0, gemm_tile = 0,
+      bc_sweeps = 0, bc_waves = 0;
+  void to(int *k) const {
+    const int v[EVD_TUNE_KNOBS] = {nb, chase_groups, bc_valu, sbr_back_block, chase_depth, panel_rows, gemm_tile,
+                                   bc_sweeps, bc_waves};
+    for (int i = 0; i < EVD_TUNE_KNOBS; i++) k[i] = v[i];
+  }
+  void from(const int *k) {
+    nb = k[0];
+    chase_groups = k[1];
+    bc_valu = k[2];
+    sbr_back_block = k[3];
+    chase_depth = k[4];
+    panel_rows = k[5];
+    gemm_tile = k[6];
+    bc_sweeps = k[7];
+    bc_waves = k[8];
+  }
+  // the ranges evd_ctx_set_knob accepts; nb = 0 or a multiple of b
+  bool valid(long b) const {
+    return nb >= 0 && (nb == 0 || (nb >= b && nb % b == 0)) && chase_groups >= 0 && chase_groups <= (1 << 20) &&
+           (bc_valu == 0 || bc_valu == 1) && sbr_back_block >= 0 && sbr_back_block <= (1 << 16) &&
+           chase_depth >= 0 && chase_depth <= 6 && (panel_rows == 0 || panel_rows == 256 || panel_rows == 512) &&
+           gemm_tile >= 0 && gemm_tile <= 2 && bc_sweeps >= 0 && bc_sweeps <= 64 && bc_waves >= 0 && bc_waves <= 16;
+  }
 };
 
 std::mutex g_mu;
@@ -51,11 +85,17 @@ bool tune_lookup(long n, long b, int precision, int *nb, Tuning *t) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cache.find(Key(bucket(n), b, precision));
   if (it == g_cache.end()) return false;
-  if (nb) *nb = it->second.nb;
+  const Entry &e = it->second;
+  if (nb) *nb = e.nb;
   if (t) {
-    t->chase_groups = it->second.chase_groups;
-    t->bc_valu = it->second.bc_valu;
-    t->sbr_back_block = it->second.sbr_back_block;
+    t->chase_groups = e.chase_groups;
+    t->bc_valu = e.bc_valu;
+    t->sbr_back_block = e.sbr_back_block;
+    t->chase_depth = e.chase_depth;
+    t->panel_rows = e.panel_rows;
+    t->gemm_tile = e.gemm_tile;
+    t->bc_sweeps = e.bc_sweeps;
+    t->bc_waves = e.bc_waves;
   }
   return true;
 }
@@ -122,13 +162,22 @@ extern "C" int evd_autotune(evd_ctx *ctx, long n, long b, int reps, int *out_kno
       if (v >= b && v <= std::max<long>(256, n)) nbs.push_back(v);
     Knob knobs[] = {
         {&nb, nbs, -1, -1},
+        {&ctx->tune.panel_rows, {0, 256}, EVD_STAGE_SBR, EVD_STAGE_SBR + 1},
+        {&ctx->tune.gemm_tile, {0, 1, 2}, -1, -1},
+        {&ctx->tune.chase_depth, {0, 2, 3}, EVD_STAGE_BC, EVD_STAGE_BC + 1},
         {&ctx->tune.chase_groups, {0, (int)(n / 128) + 4, (int)(3 * n / 256) + 4, (int)(n / 32) + 4}, EVD_STAGE_BC,
          EVD_STAGE_BC + 1},
         {&ctx->tune.bc_valu, {0, 1}, EVD_STAGE_BCBACK, EVD_STAGE_BCBACK + 1},
+        {&ctx->tune.bc_sweeps, {0, 32}, EVD_STAGE_BCBACK, EVD_STAGE_BCBACK + 1},
+        {&ctx->tune.bc_waves, {0, 4}, EVD_STAGE_BCBACK, EVD_STAGE_BCBACK + 1},
         {&ctx->tune.sbr_back_block, {0, 512, 2048}, EVD_STAGE_SBRBACK, EVD_STAGE_SBRBACK + 1},
     };
     for (Knob &kb : knobs) {
       if (kb.slot == &ctx->tune.bc_valu && b != 32) continue;  // one BC-back kernel for b = 64
+      if (kb.slot == &ctx->tune.chase_depth && b > 32) continue;  // the grouped chase needs b <= 32
+      // the VALU BC back transformation's shape, when that kernel runs
+      const bool valu_bc = (b == 64 || ctx->tune.bc_valu == 1);
+      if ((kb.slot == &ctx->tune.bc_sweeps || kb.slot == &ctx->tune.bc_waves) && !valu_bc) continue;
       int best_v = *kb.slot;
       float best_t = 1e30f;
       for (int v : kb.cand) {
@@ -152,6 +201,11 @@ extern "C" int evd_autotune(evd_ctx *ctx, long n, long b, int reps, int *out_kno
     best.chase_groups = ctx->tune.chase_groups;
     best.bc_valu = ctx->tune.bc_valu;
     best.sbr_back_block = ctx->tune.sbr_back_block;
+    best.chase_depth = ctx->tune.chase_depth;
+    best.panel_rows = ctx->tune.panel_rows;
+    best.gemm_tile = ctx->tune.gemm_tile;
+    best.bc_sweeps = ctx->tune.bc_sweeps;
+    best.bc_waves = ctx->tune.bc_waves;
   }
   hipStreamSynchronize(ctx->stream);
   if (A0) hipFree(A0);
@@ -166,12 +220,7 @@ extern "C" int evd_autotune(evd_ctx *ctx, long n, long b, int reps, int *out_kno
     std::lock_guard<std::mutex> lk(g_mu);
     g_cache[Key(bucket(n), b, ctx->precision)] = best;
   }
-  if (out_knobs) {
-    out_knobs[0] = best.nb;
-    out_knobs[1] = best.chase_groups;
-    out_knobs[2] = best.bc_valu;
-    out_knobs[3] = best.sbr_back_block;
-  }
+  if (out_knobs) best.to(out_knobs);
   return 0;
 }
 
@@ -180,13 +229,9 @@ extern "C" int evd_tune_cache_put(long n, long b, int precision, const int *knob
   if (b < 1) return -2;
   if (precision < EVD_PREC_FP32 || precision > EVD_PREC_FP16) return -3;
   if (!knobs) return -4;
-  if (knobs[0] < 0 || knobs[1] < 0 || knobs[2] < 0 || knobs[2] > 1 || knobs[3] < 0) return -4;
-  if (knobs[0] != 0 && (knobs[0] < b || knobs[0] % b != 0)) return -4;
   Entry e;
-  e.nb = knobs[0];
-  e.chase_groups = knobs[1];
-  e.bc_valu = knobs[2];
-  e.sbr_back_block = knobs[3];
+  e.from(knobs);
+  if (!e.valid(b)) return -4;
   std::lock_guard<std::mutex> lk(g_mu);
   g_cache[Key(bucket(n), b, precision)] = e;
   return 0;
@@ -197,10 +242,7 @@ extern "C" int evd_tune_cache_get(long n, long b, int precision, int *knobs) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cache.find(Key(bucket(n), b, precision));
   if (it == g_cache.end()) return 1;
-  knobs[0] = it->second.nb;
-  knobs[1] = it->second.chase_groups;
-  knobs[2] = it->second.bc_valu;
-  knobs[3] = it->second.sbr_back_block;
+  it->second.to(knobs);
   return 0;
 }
 
@@ -215,10 +257,14 @@ extern "C" int evd_tune_cache_save(const char *path) {
   FILE *f = fopen(path, "w");
   if (!f) return EVD_E_UNSUPPORTED;
   std::lock_guard<std::mutex> lk(g_mu);
-  fprintf(f, "# n_bucket b precision nb chase_groups bc_valu sbr_back_block\n");
-  for (auto &kv : g_cache)
-    fprintf(f, "%ld %ld %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
-            kv.second.nb, kv.second.chase_groups, kv.second.bc_valu, kv.second.sbr_back_block);
+  fprintf(f, "# n_bucket b precision nb chase_groups bc_valu sbr_back_block chase_depth panel_rows gemm_tile "
+             "bc_sweeps bc_waves\n");
+  for (auto &kv : g_cache) {
+    const Entry &e = kv.second;
+    fprintf(f, "%ld %ld %d %d %d %d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
+            std::get<2>(kv.first), e.nb, e.chase_groups, e.bc_valu, e.sbr_back_block, e.chase_depth, e.panel_rows,
+            e.gemm_tile, e.bc_sweeps, e.bc_waves);
+  }
   fclose(f);
   return 0;
 }
@@ -237,10 +283,10 @@ extern "C" int evd_tune_cache_load(const char *path) {
     long nbk, b;
     int prec;
     Entry e;
-    if (sscanf(line, "%ld %ld %d %d %d %d %d", &nbk, &b, &prec, &e.nb, &e.chase_groups, &e.bc_valu,
-               &e.sbr_back_block) != 7 ||
-        nbk < 1 || b < 1 || prec < EVD_PREC_FP32 || prec > EVD_PREC_FP16 || e.nb < 0 || e.chase_groups < 0 ||
-        e.bc_valu < 0 || e.bc_valu > 1 || e.sbr_back_block < 0 || (e.nb != 0 && (e.nb < b || e.nb % b != 0))) {
+    const int got = sscanf(line, "%ld %ld %d %d %d %d %d %d %d %d %d %d", &nbk, &b, &prec, &e.nb, &e.chase_groups,
+                           &e.bc_valu, &e.sbr_back_block, &e.chase_depth, &e.panel_rows, &e.gemm_tile, &e.bc_sweeps,
+                           &e.bc_waves);
+    if ((got != 7 && got != 12) || nbk < 1 || b < 1 || prec < EVD_PREC_FP32 || prec > EVD_PREC_FP16 || !e.valid(b)) {
       bad = 1;
       break;
     }

@@ -79,9 +79,15 @@ int evd_ctx_set_tuning(evd_ctx *ctx, int bc_sweeps, int bc_waves, int chase_grou
 /* Set one tuning knob by name (0 restores the default): "bc_sweeps",
  * "bc_waves", "chase_groups", "sbr_back_block", "bc_valu" (1: run the BC
  * back transformation on the FP32 VALU kernel instead of the WY blocks on the
- * matrix cores) and "skinny_split" (1..4: split-K workgroups per CU of the
- * long-K skinny SBR products, default 4).  Returns -2 for an unknown name, -3
- * for a value out of range.
+ * matrix cores), "skinny_split" (1..4: split-K workgroups per CU of the
+ * long-K skinny SBR products, default 4), "chase_depth" (1..6 sweeps per
+ * bulge-chase workgroup, at most 3 and what its LDS window holds are used;
+ * default 1), "panel_rows" (256 or 512 rows per panel-QR workgroup, default
+ * 512), "gemm_tile" (1: 128 x 128, 2: 64 x 64 output tiles of this
+ * library's FP32 products; default by shape) and "own_gemm" (1: the large
+ * plain FP32 products run on this library's MFMA kernels; by default they go
+ * to rocBLAS on the context stream, deterministic mode).  Returns -2 for an
+ * unknown name, -3 for a value out of range.
  * No counterpart in the reference (its tile shapes are compile-time). */
 int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value);
 int evd_ctx_set_precision(evd_ctx *ctx, int precision);
@@ -281,16 +287,22 @@ int evd_graph_destroy(evd_graph *g);
 
 /* ---- autotuning (no counterpart in the reference: its tile shapes are
  * compile-time template arguments and the myEVD nb argument) ----
- * Knob vector: {nb, chase_groups, bc_valu, sbr_back_block} (0 = default).
+ * Knob vector (EVD_TUNE_KNOBS entries, 0 = default): {nb, chase_groups,
+ * bc_valu, sbr_back_block, chase_depth, panel_rows, gemm_tile, bc_sweeps,
+ * bc_waves} - the SBR block and panel width (nb, panel rows per workgroup),
+ * the MFMA output tile of the large products, the chase's sweeps per
+ * workgroup and workgroup count, and the BC back transformation's kernel and
+ * sweep depth (sweeps per block, blocks in flight).
  * evd_autotune solves a synthetic matrix of order n (device buffers of 3 n^2
  * floats allocated for the call) for candidate knob values, one knob at a
  * time, and stores the fastest setting in the process-wide cache under
  * (next power of two >= n, b, the context's precision); knobs_out (nullable)
  * receives it.  evd_syevd_f32 fills every knob the context leaves at 0, and
  * nb = 0, from the cache entry of its (n, b, precision) when there is one.
- * The cache is saved / loaded as text ("n_bucket b precision nb chase_groups
- * bc_valu sbr_back_block" per line); load merges and rejects malformed files
- * as a whole.  evd_tune_cache_get returns 1 when no entry exists. */
+ * The cache is saved / loaded as text ("n_bucket b precision" and the knob
+ * vector per line; lines with only nb .. sbr_back_block load the others as
+ * 0); load merges and rejects malformed files as a whole.  evd_tune_cache_get returns 1 when no entry exists. */
+#define EVD_TUNE_KNOBS 9
 int evd_autotune(evd_ctx *ctx, long n, long b, int reps, int *knobs_out);
 int evd_tune_cache_put(long n, long b, int precision, const int *knobs);
 int evd_tune_cache_get(long n, long b, int precision, int *knobs);

@@ -36,11 +36,15 @@ def test_autotune_fills_cache_and_solves(tctx):
     knobs = E.autotune(tctx, 600, 32, reps=1)
     assert E.tune_cache_get(600, 32, E.PREC_FP32) == knobs
     assert knobs["nb"] in (256, 512, 1024) and knobs["bc_valu"] in (0, 1)
+    assert knobs["panel_rows"] in (0, 256) and knobs["gemm_tile"] in (0, 1, 2) and knobs["chase_depth"] in (0, 2, 3)
     _check(E, tctx, O.gen_symmetric(600, 11, 0), 32, 0)
 
 
-@pytest.mark.parametrize("knobs", [{"nb": 256, "chase_groups": 64, "bc_valu": 1, "sbr_back_block": 512},
-                                   {"nb": 512, "chase_groups": 256, "bc_valu": 0, "sbr_back_block": 2048}])
+@pytest.mark.parametrize("knobs", [{"nb": 256, "chase_groups": 64, "bc_valu": 1, "sbr_back_block": 512,
+                                    "bc_sweeps": 32, "bc_waves": 4},
+                                   {"nb": 512, "chase_groups": 256, "bc_valu": 0, "sbr_back_block": 2048,
+                                    "chase_depth": 3, "panel_rows": 256, "gemm_tile": 2},
+                                   {"nb": 1024, "chase_depth": 2, "gemm_tile": 1}])
 def test_cached_knobs_keep_accuracy(tctx, knobs):
     import evd4sc2025_amd as E
 

@@ -34,6 +34,41 @@ def test_sgemm_matches_fp64(ctx, ta, tb, m, n, k):
     assert err < 5e-6 * np.sqrt(k), err
 
 
+def _view(a, off, ld):
+    """Device column-major view of a at row offset `off` inside columns of
+    leading dimension `ld` (base alignment and ld chosen by the caller)."""
+    import torch
+
+    rows, cols = a.shape
+    full = np.zeros((ld, cols), dtype=a.dtype)
+    full[off:off + rows] = a
+    return torch.from_numpy(np.ascontiguousarray(full.T)).to("cuda").t()[off:off + rows]
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("off,pad", [(4, 7), (1, 3), (0, 0)])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_gemm_strided_views(ctx, ta, tb, off, pad, dt):
+    """Operands at 16-byte aligned and unaligned bases with padded leading
+    dimensions and edges that cut a 16-byte vector (m, k odd): the vector and
+    element load paths of the tile loader give the FP64 product."""
+    import evd4sc2025_amd as E
+
+    m, n, k = 301, 203, 101
+    f = np.float32 if dt == "f32" else np.float64
+    rng = np.random.default_rng(off + pad)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k)).astype(f)
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n)).astype(f)
+    C0 = rng.standard_normal((m, n)).astype(f)
+    ld = lambda r: (r + off + pad + 3) // 4 * 4 if pad else r + off
+    dA, dB, dC = _view(A, off, ld(A.shape[0])), _view(B, off, ld(B.shape[0])), to_dev(C0)
+    (E.sgemm if dt == "f32" else E.dgemm)(ctx, ta, tb, 0.75, dA, dB, -0.5, dC)
+    want = 0.75 * ((A.T if ta == "T" else A).astype(np.float64) @ (B.T if tb == "T" else B).astype(np.float64)) \
+        - 0.5 * C0
+    err = np.abs(to_host(dC) - want).max() / np.abs(want).max()
+    assert err < (5e-6 * np.sqrt(k) if dt == "f32" else 1e-13), err
+
+
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
 def test_dgemm_matches_fp64(ctx, ta, tb):
     import evd4sc2025_amd as E
@@ -172,6 +207,35 @@ def test_stedc_matches_lapack(ctx, name, d, e):
     resid = np.linalg.norm(T @ Z - Z * w[None, :]) / (scale * max(n, 1))
     assert resid < 1e-14, resid
     assert O.orthogonality(Z) < 1e-14
+
+
+@pytest.mark.parametrize("n", [4096, 6001])
+def test_stedc_library_products_match_grouped_kernel(ctx, n):
+    """Merges of order >= 4096 form their FP64 products with rocBLAS (device
+    problem list read back, product scattered to its columns); the knob
+    own_gemm = 1 keeps them on the grouped tile kernel.  Both against LAPACK,
+    and the two eigenvector sets against each other."""
+    import evd4sc2025_amd as E
+
+    rng = np.random.default_rng(n)
+    d, e = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n - 1)
+    w1, Z1 = E.stedc(ctx, to_dev(d), to_dev(e))
+    ctx.set_knob("own_gemm", 1)
+    try:
+        w2, Z2 = E.stedc(ctx, to_dev(d), to_dev(e))
+    finally:
+        ctx.set_knob("own_gemm", 0)
+    w1, Z1, w2, Z2 = to_host(w1), to_host(Z1), to_host(w2), to_host(Z2)
+    w0 = O.tridiag_eig(d, e, vectors=False)
+    scale = max(1.0, np.abs(w0).max())
+    for w in (w1, w2):
+        np.testing.assert_allclose(w, w0, atol=1e-13 * scale * n ** 0.5)
+    np.testing.assert_allclose(w1, w2, atol=1e-14 * scale)
+    assert np.abs(Z1 - Z2).max() < 1e-8
+    T_Z1 = d[:, None] * Z1
+    T_Z1[:-1] += e[:, None] * Z1[1:]
+    T_Z1[1:] += e[:, None] * Z1[:-1]
+    assert np.linalg.norm(T_Z1 - Z1 * w1[None, :]) / (scale * n) < 1e-14
 
 
 @pytest.mark.parametrize("valu", [0, 1])
