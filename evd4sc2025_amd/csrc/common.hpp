@@ -32,6 +32,7 @@ struct Tuning {
   int panel_rows = 0;      // panel QR rows per workgroup for panels <= 32 wide (0 = 512; 256)
   int gemm_tile = 0;       // output tile of the large FP32 products (0 = auto, 1 = 128 x 128, 2 = 64 x 64)
   int own_gemm = 0;        // 1: large plain FP32 products on this library's MFMA kernels, not rocBLAS
+  int panel_exchange = 0;  // 1: panels on the column-exchange kernel only (no Cholesky QR path)
 };
 
 // Merge plan of the tridiagonal divide and conquer for one order n (host
@@ -67,7 +68,7 @@ struct evd_ctx {
   unsigned *ctrl = nullptr;  // kCtrlWords words, zeroed at creation
   size_t ctrl_words = 0;
 
-  unsigned *tags = nullptr;  // kTagWords persistent exchange tags, zeroed at creation
+  unsigned *tags = nullptr;  // kTagWords persistent exchange tags + kCqCountWords counters, zeroed at creation
   evd::Tuning tune;
   bool tune_cache = true;  // fill knobs left at 0 from the process-wide tuning cache
   int precision = EVD_PREC_FP32;
@@ -109,6 +110,7 @@ enum : int {
   kCtrlBarrier = 16,   // grid barrier: [count, generation] pairs, 64 slots
   kCtrlWordsTotal = 16 + 2 * 64,
   kTagWords = 4096,
+  kCqCountWords = 4096,  // reduction counters of the Cholesky QR panel, after the tags
 };
 
 int set_error(evd_ctx *ctx, int code, const char *fmt, ...);

@@ -244,8 +244,8 @@ int evd_ctx_create(int device, void *stream, evd_ctx **out) {
   ctx->ctrl_words = kCtrlWordsTotal;
   if (hipMalloc(&ctx->ctrl, ctx->ctrl_words * sizeof(unsigned)) != hipSuccess ||
       hipMemset(ctx->ctrl, 0, ctx->ctrl_words * sizeof(unsigned)) != hipSuccess ||
-      hipMalloc(&ctx->tags, kTagWords * sizeof(unsigned)) != hipSuccess ||
-      hipMemset(ctx->tags, 0, kTagWords * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&ctx->tags, (kTagWords + kCqCountWords) * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(ctx->tags, 0, (kTagWords + kCqCountWords) * sizeof(unsigned)) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     delete ctx;
     return EVD_E_HIP;
@@ -349,7 +349,8 @@ int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value) {
                {"chase_groups", &ctx->tune.chase_groups, 0, 1 << 20}, {"sbr_back_block", &ctx->tune.sbr_back_block, 0, 1 << 16},
                {"bc_valu", &ctx->tune.bc_valu, 0, 1},           {"skinny_split", &ctx->tune.skinny_split, 0, 4},
                {"chase_depth", &ctx->tune.chase_depth, 0, 6},   {"panel_rows", &ctx->tune.panel_rows, 0, 512},
-               {"gemm_tile", &ctx->tune.gemm_tile, 0, 2},       {"own_gemm", &ctx->tune.own_gemm, 0, 1}};
+               {"gemm_tile", &ctx->tune.gemm_tile, 0, 2},       {"own_gemm", &ctx->tune.own_gemm, 0, 1},
+               {"panel_exchange", &ctx->tune.panel_exchange, 0, 1}};
   for (auto &kb : knobs)
     if (!strcmp(kb.name, name)) {
       if (value < kb.lo || value > kb.hi) return -3;

@@ -460,6 +460,9 @@ static int sgemm_library(evd_ctx *ctx, bool TA, bool TB, long m, long n, long k,
                          long batch) {
   const rocblas_operation oa = TA ? rocblas_operation_transpose : rocblas_operation_none;
   const rocblas_operation ob = TB ? rocblas_operation_transpose : rocblas_operation_none;
+  // the stream of this call (a capture swaps the context's stream)
+  if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+    return set_error(ctx, EVD_E_HIP, "rocblas_set_stream failed");
   for (long q = 0; q < batch; q++) {
     const rocblas_status st = rocblas_sgemm(ctx->blas, oa, ob, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha,
                                             A + q * sA, (rocblas_int)lda, B + q * sB, (rocblas_int)ldb, &beta,

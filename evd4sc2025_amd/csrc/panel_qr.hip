@@ -70,7 +70,8 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const T *_
                                                        T *__restrict__ Aout, long ldo, double *__restrict__ part,
                                                        double *__restrict__ tau_out, unsigned *__restrict__ tags,
                                                        unsigned *__restrict__ epoch_word, unsigned *__restrict__ err,
-                                                       long s_in, long s_y, long s_w, long s_out) {
+                                                       long s_in, long s_y, long s_w, long s_out,
+                                                       const int *__restrict__ only_bad) {
   constexpr int CH = NB < 32 ? NB : 32;  // slots per in-wave reduction
   constexpr int PASSG = 32;              // group records gathered per pass
   static_assert(NB <= 256, "the pivot rows must live in group 0");
@@ -98,6 +99,10 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const T *_
   __shared__ double gsum[256];
   __shared__ int s_ok;
 
+  // fallback of the Cholesky QR panel: problems whose state word is not 1
+  // (factored there) load nothing and leave after counting into the launch
+  // epoch below
+  const bool skip = only_bad && only_bad[blockIdx.y] != 1;
   double a[RPT][NB];
   long rowv[RPT];
 #pragma unroll
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const T *_
     const long r = (long)g * 256 * RPT + q * 256 + tid;
     rowv[q] = r;
 #pragma unroll
-    for (int c = 0; c < NB; c++) a[q][c] = (r < m && c < nc) ? (double)Ain[r + (long)c * lda] : 0.0;
+    for (int c = 0; c < NB; c++) a[q][c] = (!skip && r < m && c < nc) ? (double)Ain[r + (long)c * lda] : 0.0;
   }
   for (int i = tid; i < NB * (NB + 1); i += 256) {
     (&gram[0][0])[i] = 0.0;
@@ -133,6 +138,8 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const T *_
   }
   __syncthreads();
   const unsigned epoch = s_epoch;
+  // (every group has counted itself into the epoch above, so the word advances)
+  if (skip) return;
   if (G > 1) {
     // sentinels of steps 0 and 1 in this group's records, then a one-time
     // start barrier so nobody polls a slot before its owner marked it
@@ -388,7 +395,7 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int nc, const T *_
 template <int NB, int RPT, typename T>
 static int launch_panel(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *Y, long ldy, T *W, long ldw, T *R,
                         long ldr, T *Aout, long ldo, double *tau_out, int count = 1, long s_in = 0, long s_y = 0,
-                        long s_w = 0, long s_out = 0) {
+                        long s_w = 0, long s_out = 0, const int *only_bad = nullptr) {
   const long rows_per = 256L * RPT;
   int G = (int)((m + rows_per - 1) / rows_per);
   if (count > 1 && (R || tau_out)) return set_error(ctx, EVD_E_UNSUPPORTED, "batched panels write no R / tau output");
@@ -407,7 +414,7 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *
   unsigned *err = ctx->ctrl + kCtrlError;
   int mi = (int)m, ni = (int)nc;
   void *args[] = {&mi, &ni, (void *)&A, &lda, &Y, &ldy, &W, &ldw, &R, &ldr, &Aout, &ldo, &part, &tau_out,
-                  &tags, &epoch_word, &err, &s_in, &s_y, &s_w, &s_out};
+                  &tags, &epoch_word, &err, &s_in, &s_y, &s_w, &s_out, (void *)&only_bad};
   // one workgroup is trivially co-resident: a regular launch, which (unlike a
   // cooperative one) may overlap with the work of other streams
   if (G == 1)
@@ -419,12 +426,35 @@ static int launch_panel(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *
   return 0;
 }
 
+template <typename T>
+int panel_cholqr(evd_ctx *ctx, int count, long m, long nc, const T *A, long lda, long s_in, T *Y, long ldy, long s_y,
+                 T *W, long ldw, long s_w, T *R, long ldr, T *Aout, long ldo, long s_out, double *tau_out, int *bad);
+
+// Whether a panel goes through the Cholesky QR path (panel_cholqr.hip): up to
+// 32 columns and at least twice as many rows, unless the panel_exchange knob
+// keeps the column-exchange kernel.
+static bool use_cholqr(const evd_ctx *ctx, long m, long nc) {
+  return !ctx->tune.panel_exchange && nc <= 32 && m >= 2 * nc;
+}
+
 // Panel factorisation used by the SBR drivers (FP32 and FP64 storage; the
 // arithmetic is FP64 in both) and by evd_panel_qr_f32 / _f64.
 template <typename T>
 int panel_qr_t(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *Y, long ldy, T *W, long ldw, T *R, long ldr,
                T *Aout, long ldo, double *tau_out) {
   if (m <= 0 || nc <= 0) return 0;
+  if (use_cholqr(ctx, m, nc)) {
+    // Cholesky QR + reconstruction; the column-exchange kernel runs after it
+    // and factors the panel only if it was flagged
+    ArenaFrame frame(ctx);
+    int *bad = arena_alloc_n<int>(ctx, 1);
+    if (!bad) return EVD_E_NOMEM;
+    EVD_TRY(panel_cholqr<T>(ctx, 1, m, nc, A, lda, 0, Y, ldy, 0, W, ldw, 0, R, ldr, Aout, ldo, 0, tau_out, bad));
+    if (ctx->tune.panel_rows == 256)
+      return launch_panel<32, 1, T>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out, 1, 0, 0, 0, 0,
+                                    bad);
+    return launch_panel<32, 2, T>(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out, 1, 0, 0, 0, 0, bad);
+  }
   // rows per workgroup (the panel_rows knob): 512 (two per thread) by
   // default; 256 doubles the workgroups that share the column exchanges
   if (nc <= 32 && ctx->tune.panel_rows == 256)
@@ -449,6 +479,29 @@ int panel_qr_f64(evd_ctx *ctx, long m, long nc, const double *A, long lda, doubl
 int panel_qr_batched(evd_ctx *ctx, int count, long m, long nc, float *A, long lda, long sA, float *Y, long ldy,
                      long sY, float *W, long ldw, long sW) {
   if (m <= 0 || nc <= 0 || count <= 0) return 0;
+  if (use_cholqr(ctx, m, nc)) {
+    // every problem's panel through the Cholesky QR path in one launch each,
+    // then the fallback for the flagged ones (one launch when the groups of
+    // all problems can be co-resident, else per problem)
+    ArenaFrame frame(ctx);
+    int *bad = arena_alloc_n<int>(ctx, (size_t)count);
+    if (!bad) return EVD_E_NOMEM;
+    EVD_TRY(panel_cholqr<float>(ctx, count, m, nc, A, lda, sA, Y, ldy, sY, W, ldw, sW, nullptr, 0, A, lda, sA,
+                                nullptr, bad));
+    const long G = (m + 511) / 512;
+    int maxres = 0;
+    if (count > 1 && G > 1)
+      EVD_TRY(max_coresident_blocks(ctx, (const void *)panel_qr_kernel<32, 2, float>, 256, 0, &maxres));
+    if (count == 1 || G == 1 || (G * count <= maxres && G * count <= kTagWords))
+      return launch_panel<32, 2, float>(ctx, m, nc, A, lda, Y, ldy, W, ldw, nullptr, 0, A, lda, nullptr, count, sA,
+                                        sY, sW, sA, bad);
+    for (int q = 0; q < count; q++) {
+      const int st = launch_panel<32, 2, float>(ctx, m, nc, A + q * sA, lda, Y + q * sY, ldy, W + q * sW, ldw,
+                                                nullptr, 0, A + q * sA, lda, nullptr, 1, 0, 0, 0, 0, bad + q);
+      if (st) return st;
+    }
+    return 0;
+  }
   const long rows_per = nc <= 32 ? 512 : 256;
   const long G = (m + rows_per - 1) / rows_per;
   int maxres = 0;

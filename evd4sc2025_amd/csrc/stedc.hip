@@ -896,6 +896,8 @@ static int dc_level_library(evd_ctx *ctx, const DGemmProblem *dprob, int nprob, 
   EVD_HIP(ctx, hipMemcpyAsync(hp.data(), dprob, sizeof(DGemmProblem) * nprob, hipMemcpyDeviceToHost, ctx->stream));
   EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const double one = 1.0, zero = 0.0;
+  if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+    return set_error(ctx, EVD_E_HIP, "rocblas_set_stream failed");
   for (const DGemmProblem &p : hp) {
     if (p.m <= 0 || p.n <= 0) continue;
     if ((size_t)p.m * p.n > t_words)

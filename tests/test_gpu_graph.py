@@ -21,8 +21,10 @@ def own_ctx():
 
 
 # n = 256: single-workgroup panels; n = 700 and 1024: panels of several
-# workgroups (cooperative launches inside the graph), odd trailing sizes
-@pytest.mark.parametrize("n,jobz", [(256, "V"), (256, "N"), (700, "V"), (1024, "V"), (1024, "N")])
+# workgroups (cooperative launches inside the graph), odd trailing sizes;
+# n = 2304: the SBR back transformation's first block product (2272 x 2304 x
+# 1024) runs on the library GEMM, which the capture records on its stream
+@pytest.mark.parametrize("n,jobz", [(256, "V"), (256, "N"), (700, "V"), (1024, "V"), (1024, "N"), (2304, "V")])
 def test_graph_replays_equal_eager(own_ctx, n, jobz):
     import evd4sc2025_amd as E
 

@@ -136,6 +136,72 @@ def test_panel_qr_matches_oracle(ctx, m, b):
     assert np.abs(rec - P).max() < 1e-5 * scale * np.sqrt(m)
 
 
+def _panel_case(kind, m, b, seed):
+    rng = np.random.default_rng(seed)
+    P = rng.uniform(-1, 1, (m, b))
+    if kind == "dup":  # rank deficient: two repeated columns
+        P[:, 5] = P[:, 2]
+        P[:, 20] = -P[:, 7]
+    elif kind == "zero":
+        P[:] = 0.0
+    elif kind == "illcond":  # full rank, condition number about 1e9
+        U, _ = np.linalg.qr(rng.standard_normal((m, b)))
+        Vt, _ = np.linalg.qr(rng.standard_normal((b, b)))
+        P = (U * np.logspace(0, -9, b)) @ Vt
+    elif kind == "scaled":  # columns of very different norms, well conditioned
+        P = P * np.logspace(3, -3, b)
+    elif kind == "moderate":  # condition number 1e4: two Cholesky passes
+        U, _ = np.linalg.qr(rng.standard_normal((m, b)))
+        Vt, _ = np.linalg.qr(rng.standard_normal((b, b)))
+        P = (U * np.logspace(0, -4, b)) @ Vt
+    return P.astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["random", "dup", "zero", "illcond", "scaled", "moderate"])
+@pytest.mark.parametrize("m", [64, 700, 5000])
+def test_panel_cholqr_and_fallback_match_oracle(ctx, kind, m):
+    """Panels of 32 columns go through Cholesky QR + Householder
+    reconstruction (one Cholesky pass for well-conditioned FP32 panels, two
+    for condition numbers above 1e3); rank-deficient and ill-conditioned ones
+    are flagged on the device and factored by the column-exchange kernel.  Every kind against the
+    dgeqrf-pinned oracle, and the Cholesky path against the exchange path
+    (knob panel_exchange = 1)."""
+    import evd4sc2025_amd as E
+
+    torch = _torch()
+    b = 32
+    P = _panel_case(kind, m, b, 70 + m)
+    V, T, R, tau = O.panel_qr(P.astype(np.float64))
+    outs = []
+    for exch in (0, 1):
+        ctx.set_knob("panel_exchange", exch)
+        try:
+            dA, dW, dR = to_dev(P), E.colmajor(m, b, fill=0.0), E.colmajor(b, b, fill=0.0)
+            E.panelQR(ctx, dA, dW, dR)
+            torch.cuda.synchronize()
+        finally:
+            ctx.set_knob("panel_exchange", 0)
+        outs.append((to_host(dA), to_host(dW), np.triu(to_host(dR))))
+    scale = max(1.0, np.abs(P).max())
+    for Y, W, Rg in outs:
+        assert np.all(np.isfinite(Y)) and np.all(np.isfinite(W)) and np.all(np.isfinite(Rg))
+        # the factorisation reproduces the panel: (I - W Y^T)[R; 0] = P
+        Rf = np.zeros((m, b))
+        Rf[:b] = Rg
+        rec = Rf - W.astype(np.float64) @ (Y.astype(np.float64).T @ Rf)
+        assert np.abs(rec - P).max() < 1e-5 * scale * np.sqrt(m), kind
+        # Q = I - W Y^T is orthogonal on the panel's span
+        Qt = np.eye(m)[:, :b] - W.astype(np.float64) @ Y.astype(np.float64)[:b].T
+        assert np.abs(Qt.T @ Qt - np.eye(b)).max() < 1e-4, kind
+        if kind in ("random", "scaled", "moderate"):
+            np.testing.assert_allclose(Y, V, atol=2e-5 * np.sqrt(m))
+            np.testing.assert_allclose(W, V @ T, atol=2e-5 * np.sqrt(m))
+            np.testing.assert_allclose(Rg, R[:b], atol=2e-5 * scale * np.sqrt(m))
+    if kind in ("random", "scaled", "moderate"):
+        (Y0, W0, R0), (Y1, W1, R1) = outs
+        assert np.abs(Y0 - Y1).max() < 1e-5 and np.abs(R0 - R1).max() < 1e-5 * scale
+
+
 @pytest.mark.parametrize("n,b,nb", [(64, 32, 32), (130, 32, 64), (512, 32, 128), (700, 64, 128), (1000, 32, 1024)])
 def test_sy2sb_band_spectrum(ctx, n, b, nb):
     import evd4sc2025_amd as E
