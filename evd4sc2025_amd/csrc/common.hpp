@@ -171,6 +171,10 @@ int copy_f32(evd_ctx *ctx, const float *src, long lds, float *dst, long ldd, lon
 int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, const float *A, long lda,
           const float *B, long ldb, float beta, float *C, long ldc, long batch = 1, long sA = 0, long sB = 0,
           long sC = 0);
+int sgemm_kcat(evd_ctx *ctx, char ta, char tb, long m, long n, long k1, long k2, float alpha, const float *A1,
+               long lda1, const float *B1, long ldb1, const float *A2, long lda2, const float *B2, long ldb2, float beta,
+               float *C, long ldc, long batch = 1, long sA1 = 0, long sB1 = 0, long sA2 = 0, long sB2 = 0,
+               long sC = 0);
 int dgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, double alpha, const double *A, long lda,
           const double *B, long ldb, double beta, double *C, long ldc);
 int syr2k_lower_mirror(evd_ctx *ctx, long n, long k, float alpha, const float *A, long lda, const float *B, long ldb,

@@ -253,28 +253,49 @@ __device__ inline f64x4 mfma16(double a, double b, f64x4 c) {
 }
 
 // Core: accumulate op(A)[m0:m0+BM, kb:ke] * op(B)[kb:ke, n0:n0+BN] into acc.
+// Optional second operand pair of a K-concatenated product
+// op(A) op(B) + op(A2) op(B2): K indices from K1 on (a multiple of BK) read
+// A2 / B2 at k - K1.
+template <typename T>
+struct KPair {
+  const T *A2 = nullptr;
+  long lda2 = 0;
+  const T *B2 = nullptr;
+  long ldb2 = 0;
+  int K1 = 0x7fffffff;
+};
+
 template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, typename VT, int BK = kBK>
 __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M,
-                                     int N, int m0, int n0, int kb, int ke, T *As0, T *Bs0, VT (&acc)[BM / WM / 16][BN / WN / 16]) {
+                                     int N, int m0, int n0, int kb, int ke, T *As0, T *Bs0, VT (&acc)[BM / WM / 16][BN / WN / 16],
+                                     KPair<T> kp = KPair<T>()) {
   using L = TileLoader<TA, TB, BM, BN, WM, WN, T, BK>;
   constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
   constexpr int SA = BK * L::LDSA, SB = BK * L::LDSB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   L ld;
-  // 16-byte loads where the operand's alignment allows them
+  // 16-byte loads where the operands' alignment allows them
   constexpr int VW = L::VW;
-  const bool va = ((reinterpret_cast<unsigned long>(A) & 15) == 0) && (lda % VW == 0);
-  const bool vb = ((reinterpret_cast<unsigned long>(B) & 15) == 0) && (ldb % VW == 0);
+  auto vec_ok = [](const T *p, long ld) { return ((reinterpret_cast<unsigned long>(p) & 15) == 0) && (ld % VW == 0); };
+  const bool va = vec_ok(A, lda) && (!kp.A2 || vec_ok(kp.A2, kp.lda2));
+  const bool vb = vec_ok(B, ldb) && (!kp.A2 || vec_ok(kp.B2, kp.ldb2));
+  // the K slab at k0 from the pair it lies in
+  auto load_slab = [&](int k0) {
+    if (k0 < kp.K1)
+      ld.load(A, lda, B, ldb, M, N, m0, n0, k0, min(ke, kp.K1), tid, va, vb);
+    else
+      ld.load(kp.A2, kp.lda2, kp.B2, kp.ldb2, M, N, m0, n0, k0 - kp.K1, ke - kp.K1, tid, va, vb);
+  };
   int nk = ke > kb ? (ke - kb + BK - 1) / BK : 0;
   if (nk > 0) {
-    ld.load(A, lda, B, ldb, M, N, m0, n0, kb, ke, tid, va, vb);
+    load_slab(kb);
     ld.store(As0, Bs0, tid, va, vb);
   }
   __syncthreads();
   for (int t = 0; t < nk; t++) {
     const int cur = t & 1;
-    if (t + 1 < nk) ld.load(A, lda, B, ldb, M, N, m0, n0, kb + (t + 1) * BK, ke, tid, va, vb);
+    if (t + 1 < nk) load_slab(kb + (t + 1) * BK);
     const T *As = As0 + cur * SA;
     const T *Bs = Bs0 + cur * SB;
 #pragma unroll
@@ -299,7 +320,8 @@ template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK>
 __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float alpha, const float *__restrict__ A,
                                                     long lda, long sA, const float *__restrict__ B, long ldb, long sB,
                                                     float beta, float *__restrict__ C, long ldc, long sC, int kchunk,
-                                                    int ksplit, float *__restrict__ P, long batches) {
+                                                    int ksplit, float *__restrict__ P, long batches,
+                                                    KPair<float> kp, long sA2, long sB2) {
   using L = TileLoader<TA, TB, BM, BN, WM, WN, float, BK>;
   constexpr int MI = BM / WM / 16, NI = BN / WN / 16;
   __shared__ __align__(16) float As[2 * BK * L::LDSA];
@@ -310,6 +332,10 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
   A += batch * sA;
   B += batch * sB;
   C += batch * sC;
+  if (kp.A2) {
+    kp.A2 += batch * sA2;
+    kp.B2 += batch * sB2;
+  }
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kb = kz * kchunk;
   const int ke = min(K, kb + kchunk);
@@ -318,7 +344,7 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
   for (int mi = 0; mi < MI; mi++)
 #pragma unroll
     for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
-  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK>(A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc);
+  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK>(A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc, kp);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -359,10 +385,16 @@ __global__ void splitk_reduce_kernel(int M, int N, long batches, int ksplit, con
   }
 }
 
+// The second operand pair of a K-concatenated product and its batch strides.
+struct KCat {
+  KPair<float> kp;
+  long sA2 = 0, sB2 = 0;
+};
+
 template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK>
 static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
                         const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
-                        long min_chunk = 256, long target_per_cu = 2) {
+                        long min_chunk = 256, long target_per_cu = 2, const KCat &kc = KCat()) {
   long tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN) * shape_batch(ctx, batch);
   int ksplit = 1;
   const long target = target_per_cu * ctx->num_cus;
@@ -386,7 +418,8 @@ static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const
   }
   dim3 grid((unsigned)((m + BM - 1) / BM), (unsigned)((n + BN - 1) / BN), (unsigned)(batch * ksplit));
   sgemm_kernel<TA, TB, BM, BN, WM, WN, BK><<<grid, 256, 0, ctx->stream>>>(
-      (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch);
+      (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch, kc.kp,
+      kc.sA2, kc.sB2);
   EVD_LAUNCHED(ctx);
   if (ksplit > 1) {
     long total = m * n * batch;
@@ -399,7 +432,8 @@ static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const
 
 template <bool TA, bool TB>
 static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, const float *A, long lda, long sA,
-                          const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch) {
+                          const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC, long batch,
+                          const KCat &kc = KCat()) {
   const long target = 2L * ctx->num_cus;
   if (n <= 32 && m > 64) {
     // tall-skinny: the row tile is chosen for parallelism first (the K
@@ -411,20 +445,24 @@ static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, con
     const long t256 = (m + 255) / 256 * hb, t128 = (m + 127) / 128 * hb;
     if (t256 >= target / 2 || k >= 4096)
       return launch_sgemm<TA, TB, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
-                                                 256, ctx->tune.skinny_split > 0 ? ctx->tune.skinny_split : 4);
+                                                 256, ctx->tune.skinny_split > 0 ? ctx->tune.skinny_split : 4, kc);
     if (t128 >= target / 2)
       return launch_sgemm<TA, TB, 128, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
-                                                 128);
-    return launch_sgemm<TA, TB, 64, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128);
+                                                 128, 2, kc);
+    return launch_sgemm<TA, TB, 64, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128,
+                                              2, kc);
   }
   if (m <= 32 && n > 64)
-    return launch_sgemm<TA, TB, 32, 256, 1, 4>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    return launch_sgemm<TA, TB, 32, 256, 1, 4>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                                               256, 2, kc);
   long big_tiles = ((m + 127) / 128) * ((n + 127) / 128) * shape_batch(ctx, batch);
   // the gemm_tile knob forces 128 x 128 (1) or 64 x 64 (2) output tiles
   if (ctx->tune.gemm_tile == 1 ||
       (ctx->tune.gemm_tile == 0 && (big_tiles >= target || (m > 64 && n > 64 && k < 1024))))
-    return launch_sgemm<TA, TB, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
-  return launch_sgemm<TA, TB, 64, 64, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128);
+    return launch_sgemm<TA, TB, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                                                256, 2, kc);
+  return launch_sgemm<TA, TB, 64, 64, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128,
+                                            2, kc);
 }
 
 __global__ void scale_kernel(int M, int N, float beta, float *C, long ldc, long sC, long batch) {
@@ -497,6 +535,39 @@ int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, c
   if (TA && !TB) return sgemm_dispatch<true, false>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   if (!TA && TB) return sgemm_dispatch<false, true>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   return sgemm_dispatch<true, true>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+// C = alpha (op(A1) op(B1) + op(A2) op(B2)) + beta C, the two products (inner
+// dimensions k1 and k2, same transposition) summed in one K loop: one launch
+// and one K split instead of two.  Falls back to two products when k1 is not
+// a multiple of the K slab, for the split-precision modes and for products
+// that go to the library.
+int sgemm_kcat(evd_ctx *ctx, char ta, char tb, long m, long n, long k1, long k2, float alpha, const float *A1,
+               long lda1, const float *B1, long ldb1, const float *A2, long lda2, const float *B2, long ldb2, float beta,
+               float *C, long ldc, long batch, long sA1, long sB1, long sA2, long sB2, long sC) {
+  if (m <= 0 || n <= 0 || batch <= 0) return 0;
+  const long k = k1 + k2;
+  const bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+  if (k1 <= 0 || k2 <= 0 || k1 % kBK != 0 || alpha == 0.f || ctx->precision != EVD_PREC_FP32 ||
+      library_gemm(ctx, m, n, k) || k >= (1L << 31)) {
+    EVD_TRY(sgemm(ctx, ta, tb, m, n, k1, alpha, A1, lda1, B1, ldb1, beta, C, ldc, batch, sA1, sB1, sC));
+    return sgemm(ctx, ta, tb, m, n, k2, alpha, A2, lda2, B2, ldb2, 1.f, C, ldc, batch, sA2, sB2, sC);
+  }
+  KCat kc;
+  kc.kp.A2 = A2;
+  kc.kp.lda2 = lda2;
+  kc.kp.B2 = B2;
+  kc.kp.ldb2 = ldb2;
+  kc.kp.K1 = (int)k1;
+  kc.sA2 = sA2;
+  kc.sB2 = sB2;
+  if (!TA && !TB)
+    return sgemm_dispatch<false, false>(ctx, m, n, k, alpha, A1, lda1, sA1, B1, ldb1, sB1, beta, C, ldc, sC, batch, kc);
+  if (TA && !TB)
+    return sgemm_dispatch<true, false>(ctx, m, n, k, alpha, A1, lda1, sA1, B1, ldb1, sB1, beta, C, ldc, sC, batch, kc);
+  if (!TA && TB)
+    return sgemm_dispatch<false, true>(ctx, m, n, k, alpha, A1, lda1, sA1, B1, ldb1, sB1, beta, C, ldc, sC, batch, kc);
+  return sgemm_dispatch<true, true>(ctx, m, n, k, alpha, A1, lda1, sA1, B1, ldb1, sB1, beta, C, ldc, sC, batch, kc);
 }
 
 // ----------------------------------------------------------------- SYR2K

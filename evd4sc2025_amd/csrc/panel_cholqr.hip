@@ -243,11 +243,12 @@ __global__ __launch_bounds__(256) void cq_gram_kernel(int m, int nc, const T *__
 }
 
 // Upper Cholesky factor of the 32 x 32 matrix G (LDS, pitch kLD) into R (LDS),
-// on one wave in left-looking (Crout) order: lane c keeps column c of R in
-// registers and reads the finished rows k < j back from LDS as broadcasts
-// (lanes 32..63 repeat lanes 0..31 and store nothing).  Returns false when a
-// pivot is not above tol_rel times the largest diagonal entry (or is not
-// finite).
+// on one wave, right-looking: lane c keeps working column c in registers and
+// takes the pivot and row j of R from the other lanes' registers (readlane),
+// so a step's dependent chain is the pivot, its square root and reciprocal,
+// and one multiply-add of the next pivot column; lanes 32..63 repeat lanes
+// 0..31 and store nothing.  Returns false when a pivot is not above tol_rel
+// times the largest diagonal entry (or is not finite).
 __device__ __forceinline__ bool wave_cholesky(const double *G, double *R, int lane, double tol_rel) {
   const int c = lane & (kQ - 1);
   double dmax = G[c * kLD + c];
@@ -255,24 +256,19 @@ __device__ __forceinline__ bool wave_cholesky(const double *G, double *R, int la
   for (int o = 16; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
   const double tol = tol_rel * dmax;
   bool ok = dmax > 0.0 && dmax < 1e300;
-  double r[kQ];
+  double col[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; i++) col[i] = G[i * kLD + c];
 #pragma unroll
   for (int j = 0; j < kQ; j++) {
-    double s0 = G[j * kLD + c], s1 = 0.0;  // two chains of the dot product
-#pragma unroll
-    for (int k = 0; k < j; k++) {
-      if (k & 1)
-        s1 -= R[k * kLD + j] * r[k];
-      else
-        s0 -= R[k * kLD + j] * r[k];
-    }
-    const double s = s0 + s1;
-    const double d = lane_value(s, j);  // G[j][j] - sum_k R[k][j]^2
+    const double d = lane_value(col[j], j);  // working G[j][j]
     if (!(d > tol)) ok = false;
-    const double rjj = ok ? sqrt(d) : 1.0;
-    r[j] = (c >= j) ? s / rjj : 0.0;
-    if (lane < kQ) R[j * kLD + c] = r[j];
-    lds_order();
+    const double rinv = 1.0 / (ok ? sqrt(d) : 1.0);
+    const double rjc = (c >= j) ? col[j] * rinv : 0.0;  // R[j][c]
+    if (lane < kQ) R[j * kLD + c] = rjc;
+    // trailing update G[i][c] -= R[j][i] R[j][c], i > j
+#pragma unroll
+    for (int i = j + 1; i < kQ; i++) col[i] -= lane_value(rjc, i) * rjc;
   }
   return ok;
 }
@@ -280,18 +276,22 @@ __device__ __forceinline__ bool wave_cholesky(const double *G, double *R, int la
 // X = U^-1 for an upper triangular 32 x 32 U on one wave: lane c forms column
 // c by right-looking back substitution (x_k is final once the rows below k
 // are applied, and the updates of the rows above it are independent, so the
-// dependent chain is one multiply-add per step).  UNIT_LT: U is the transpose
-// of the unit lower triangle stored in M (U[i][k] = M[k][i], unit diagonal);
-// otherwise U is M's upper triangle.
+// dependent chain is one multiply-add per step; the diagonal's reciprocals
+// are formed up front).  UNIT_LT: U is the transpose of the unit lower
+// triangle stored in M (U[i][k] = M[k][i], unit diagonal); otherwise U is
+// M's upper triangle.
 template <bool UNIT_LT>
 __device__ __forceinline__ void wave_upper_inverse(const double *M, double *X, int lane) {
   const int c = lane & (kQ - 1);
-  double s[kQ];
+  double s[kQ], rd[kQ];
 #pragma unroll
-  for (int i = 0; i < kQ; i++) s[i] = (i == c) ? 1.0 : 0.0;
+  for (int i = 0; i < kQ; i++) {
+    s[i] = (i == c) ? 1.0 : 0.0;
+    rd[i] = UNIT_LT ? 1.0 : 1.0 / M[i * kLD + i];
+  }
 #pragma unroll
   for (int k = kQ - 1; k >= 0; k--) {
-    const double xk = UNIT_LT ? s[k] : s[k] / M[k * kLD + k];
+    const double xk = s[k] * rd[k];
     s[k] = xk;
 #pragma unroll
     for (int i = 0; i < k; i++) s[i] -= (UNIT_LT ? M[k * kLD + i] : M[i * kLD + k]) * xk;
@@ -346,7 +346,6 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
   fac += (long)q * kFactorWords;
   A += q * s_in;
   __shared__ double G[kQ * kLD], R[kQ * kLD], X[kQ * kLD], Y[kQ * kLD], Z[kQ * kLD], U[kQ * kLD];
-  __shared__ double rowbuf[kQ];
   __shared__ double sgn[kQ];
   __shared__ double red[4];
   __shared__ int s_ok;
@@ -431,8 +430,8 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
   block_matmul(X, G, Z, tid);
   __syncthreads();
   // ---- sign-chosen LU of Q_top - S on wave 0, lane i holding row i (the
-  // pivot row goes through LDS): Z ends as U' (upper, diagonal included) and
-  // L (strictly lower; unit diagonal)
+  // pivot row is read from lane j's registers): Z ends as U' (upper, diagonal
+  // included) and L (strictly lower; unit diagonal)
   if (w == 0) {
     const int i = lane & (kQ - 1);
     double row[kQ];
@@ -440,24 +439,22 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
     for (int c = 0; c < kQ; c++) row[c] = Z[i * kLD + c];
 #pragma unroll
     for (int j = 0; j < kQ; j++) {
-      if (lane == j) {
-#pragma unroll
-        for (int c = j; c < kQ; c++) rowbuf[c] = row[c];
-      }
-      lds_order();
-      const double piv = rowbuf[j];
+      const double piv = lane_value(row[j], j);
       const double sj = (piv >= 0.0) ? -1.0 : 1.0;
       const double ujj = piv - sj;  // |ujj| >= 1
+      const double uinv = 1.0 / ujj;
+      double u[kQ];  // row j of U' (wave-uniform)
+#pragma unroll
+      for (int c = j + 1; c < kQ; c++) u[c] = lane_value(row[c], j);
       if (i > j) {
-        const double l = row[j] / ujj;
+        const double l = row[j] * uinv;
         row[j] = l;
 #pragma unroll
-        for (int c = j + 1; c < kQ; c++) row[c] -= l * rowbuf[c];
+        for (int c = j + 1; c < kQ; c++) row[c] -= l * u[c];
       } else if (i == j) {
         row[j] = ujj;
       }
       if (lane == j) sgn[j] = sj;
-      lds_order();
     }
     if (lane < kQ) {
 #pragma unroll

@@ -28,10 +28,19 @@ int panel_qr(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, 
 int panel_qr_batched(evd_ctx *ctx, int count, long m, long nc, float *A, long lda, long sA, float *Y, long ldy,
                      long sY, float *W, long ldw, long sW);
 
+// GEMM launches carry d problems' products shaped as one problem's (the
+// tile and K-split choice, hence the summation order, of a single product)
+struct ShapeDiv {
+  evd_ctx *c;
+  long saved;
+  ShapeDiv(evd_ctx *c_, long d) : c(c_), saved(c_->shape_div) { c->shape_div = d; }
+  ~ShapeDiv() { c->shape_div = saved; }
+};
+
 size_t sy2sb_workspace(long n, long b, long nb) {
   size_t s = 0;
   s += align256((size_t)n * nb * sizeof(float));      // Z
-  s += align256((size_t)nb * b * sizeof(float)) * 2;  // small products
+  s += align256((size_t)nb * b * sizeof(float)) * 3;  // small products
   s += (size_t)96 << 20;                              // split-K partials, panel partials
   return s;
 }
@@ -44,7 +53,7 @@ int sy2sb(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, l
   ArenaFrame frame(ctx);
   const long ldz = n;
   float *Z = arena_alloc_n<float>(ctx, (size_t)n * nb);
-  float *t = arena_alloc_n<float>(ctx, (size_t)nb * b);
+  float *t = arena_alloc_n<float>(ctx, (size_t)2 * nb * b);  // [Z_q^T W_p; Y_q^T W_p]
   float *t3 = arena_alloc_n<float>(ctx, (size_t)b * b);
   if (!Z || !t || !t3) return EVD_E_NOMEM;
   const long P = nb / b;
@@ -61,13 +70,22 @@ int sy2sb(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, l
       // Z_p = A[r0:, r0:] W_p  (A here is the level-start trailing matrix)
       EVD_TRY(sgemm(ctx, 'N', 'N', m, b, m, 1.f, A + r0 + r0 * lda, lda, Wp, ldw, 0.f, Zp, ldz));
       if (p > 0) {
+        // Z_p -= Y_q (Z_q^T W_p) + Z_q (Y_q^T W_p): the two small products in
+        // one launch (a batch of two over Z_q and Y_q, shaped as one product),
+        // the update as one K-concatenated product
         const long pb = p * b;
         const float *Yq = Y + r0 + c0 * ldy;
         const float *Zq = Z + r0;
-        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, t, pb));
-        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, pb, -1.f, Yq, ldy, t, pb, 1.f, Zp, ldz));
-        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, t, pb));
-        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, pb, -1.f, Zq, ldz, t, pb, 1.f, Zp, ldz));
+        float *tz = t, *ty = t + pb * b;
+        if (ldy == ldz) {
+          ShapeDiv pair(ctx, 2);
+          const long dYZ = (long)(reinterpret_cast<intptr_t>(Yq) - reinterpret_cast<intptr_t>(Zq)) / (long)sizeof(float);
+          EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pb, 2, dYZ, 0, pb * b));
+        } else {
+          EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pb));
+          EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, ty, pb));
+        }
+        EVD_TRY(sgemm_kcat(ctx, 'N', 'N', m, b, pb, pb, -1.f, Yq, ldy, tz, pb, Zq, ldz, ty, pb, 1.f, Zp, ldz));
       }
       EVD_TRY(sgemm(ctx, 'T', 'N', b, b, m, 1.f, Wp, ldw, Zp, ldz, 0.f, t3, b));
       EVD_TRY(sgemm(ctx, 'N', 'N', m, b, b, -0.5f, Yp, ldy, t3, b, 1.f, Zp, ldz));
@@ -76,10 +94,10 @@ int sy2sb(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, l
       const float *Za = Z + r0;
       const bool more = (p + 1 < P) && (n - (r0 + b) > 1);
       if (more) {
-        // lazy update of the next panel's b columns inside the level
+        // lazy update of the next panel's b columns inside the level:
+        // A_n -= Y_a Z_a^T + Z_a Y_a^T as one K-concatenated product
         float *An = A + r0 + r0 * lda;
-        EVD_TRY(sgemm(ctx, 'N', 'T', m, b, kk, -1.f, Ya, ldy, Za, ldz, 1.f, An, lda));
-        EVD_TRY(sgemm(ctx, 'N', 'T', m, b, kk, -1.f, Za, ldz, Ya, ldy, 1.f, An, lda));
+        EVD_TRY(sgemm_kcat(ctx, 'N', 'T', m, b, kk, kk, -1.f, Ya, ldy, Za, ldz, Za, ldz, Ya, ldy, 1.f, An, lda));
         continue;
       }
       // level end: two-sided update of the whole trailing matrix
@@ -106,7 +124,7 @@ size_t gemm_partial_words(long n) {
 }
 
 size_t sy2sb_batched_workspace(int count, long n, long b, long nb) {
-  size_t per = align256((size_t)n * nb * sizeof(float)) + align256((size_t)nb * b * sizeof(float)) * 2;
+  size_t per = align256((size_t)n * nb * sizeof(float)) + align256((size_t)nb * b * sizeof(float)) * 3;
   size_t part = align256(gemm_partial_words(n) * sizeof(float));
   return (per + part) * (size_t)count + ((size_t)8 << 20);
 }
@@ -120,13 +138,8 @@ int sy2sb_batched(evd_ctx *ctx, int count, long n, long b, long nb, float *A, lo
     EVD_TRY(fill_f32(ctx, Y + q * sY, n, wcols, ldy, 0.f));
   }
   ArenaFrame frame(ctx);
-  struct ShapeDiv {
-    evd_ctx *c;
-    long saved;
-    ShapeDiv(evd_ctx *c_, long d) : c(c_), saved(c_->shape_div) { c->shape_div = d; }
-    ~ShapeDiv() { c->shape_div = saved; }
-  } shape(ctx, count);
-  const long ldz = n, sZ = n * nb, sT = nb * b, sT3 = b * b;
+  ShapeDiv shape(ctx, count);
+  const long ldz = n, sZ = n * nb, sT = 2 * nb * b, sT3 = b * b;
   float *Z = arena_alloc_n<float>(ctx, (size_t)sZ * count);
   float *t = arena_alloc_n<float>(ctx, (size_t)sT * count);
   float *t3 = arena_alloc_n<float>(ctx, (size_t)sT3 * count);
@@ -147,10 +160,11 @@ int sy2sb_batched(evd_ctx *ctx, int count, long n, long b, long nb, float *A, lo
         const long pb = p * b;
         const float *Yq = Y + r0 + c0 * ldy;
         const float *Zq = Z + r0;
-        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, t, pb, count, sZ, sW, sT));
-        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, pb, -1.f, Yq, ldy, t, pb, 1.f, Zp, ldz, count, sY, sT, sZ));
-        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, t, pb, count, sY, sW, sT));
-        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, pb, -1.f, Zq, ldz, t, pb, 1.f, Zp, ldz, count, sZ, sT, sZ));
+        float *tz = t, *ty = t + pb * b;
+        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pb, count, sZ, sW, sT));
+        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, ty, pb, count, sY, sW, sT));
+        EVD_TRY(sgemm_kcat(ctx, 'N', 'N', m, b, pb, pb, -1.f, Yq, ldy, tz, pb, Zq, ldz, ty, pb, 1.f, Zp, ldz, count, sY,
+                           sT, sZ, sT, sZ));
       }
       EVD_TRY(sgemm(ctx, 'T', 'N', b, b, m, 1.f, Wp, ldw, Zp, ldz, 0.f, t3, b, count, sW, sZ, sT3));
       EVD_TRY(sgemm(ctx, 'N', 'N', m, b, b, -0.5f, Yp, ldy, t3, b, 1.f, Zp, ldz, count, sY, sT3, sZ));
@@ -160,8 +174,8 @@ int sy2sb_batched(evd_ctx *ctx, int count, long n, long b, long nb, float *A, lo
       const bool more = (p + 1 < P) && (n - (r0 + b) > 1);
       if (more) {
         float *An = A + r0 + r0 * lda;
-        EVD_TRY(sgemm(ctx, 'N', 'T', m, b, kk, -1.f, Ya, ldy, Za, ldz, 1.f, An, lda, count, sY, sZ, sA));
-        EVD_TRY(sgemm(ctx, 'N', 'T', m, b, kk, -1.f, Za, ldz, Ya, ldy, 1.f, An, lda, count, sZ, sY, sA));
+        EVD_TRY(sgemm_kcat(ctx, 'N', 'T', m, b, kk, kk, -1.f, Ya, ldy, Za, ldz, Za, ldz, Ya, ldy, 1.f, An, lda, count,
+                           sY, sZ, sZ, sY, sA));
         continue;
       }
       EVD_TRY(syr2k_lower_mirror_batched(ctx, count, m, kk, -1.f, Ya, ldy, sY, Za, ldz, sZ, 1.f, A + r0 + r0 * lda,
@@ -248,12 +262,7 @@ int sbr_back_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float 
   const long blk = sbr_back_blk(ctx->tune.sbr_back_block, b, wcols);
   if (blk != b && wcols % blk != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "W/Y column padding %ld not a multiple of %ld", wcols, blk);
   ArenaFrame frame(ctx);
-  struct ShapeDiv {
-    evd_ctx *c;
-    long saved;
-    ShapeDiv(evd_ctx *c_, long d) : c(c_), saved(c_->shape_div) { c->shape_div = d; }
-    ~ShapeDiv() { c->shape_div = saved; }
-  } shape(ctx, count);
+  ShapeDiv shape(ctx, count);
   const long sW = ldw * wcols, sY = ldy * wcols;
   // merges: the pairs of every problem are one batch (pair p of problem q is
   // pair q * npairs + p at stride 2 cw ld)
