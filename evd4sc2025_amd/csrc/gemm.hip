@@ -370,16 +370,27 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
   }
 }
 
+// Sum of the K-split partials in split order (eight loads in flight, added
+// one after another, so the rounding is that of a plain sequential sum).
 __global__ void splitk_reduce_kernel(int M, int N, long batches, int ksplit, const float *__restrict__ P, float alpha,
                                      float beta, float *__restrict__ C, long ldc, long sC) {
-  long total = (long)M * N * batches;
+  const long mn = (long)M * N, total = mn * batches, stride = batches * mn;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-    long b = t / ((long)M * N);
-    long rc = t % ((long)M * N);
+    long b = t / mn;
+    long rc = t % mn;
     int row = (int)(rc % M);
     long col = rc / M;
+    const float *p = P + b * mn + rc;
     float s = 0.f;
-    for (int kz = 0; kz < ksplit; kz++) s += P[((long)kz * batches + b) * (long)M * N + rc];
+    int kz = 0;
+    for (; kz + 8 <= ksplit; kz += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = p[(long)(kz + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; kz < ksplit; kz++) s += p[(long)kz * stride];
     long o = b * sC + row + col * ldc;
     C[o] = (beta == 0.f) ? alpha * s : alpha * s + beta * C[o];
   }
