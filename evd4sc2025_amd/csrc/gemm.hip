@@ -226,6 +226,45 @@ struct TileLoader {
     }
   }
 
+  // A slab that lies wholly inside both operands (the caller checks the tile,
+  // the K range and the 16-byte alignment of A): the same register mapping as
+  // load(..., va = true, vb) with every bounds test and branch left out, so a
+  // step issues a fixed number of memory operations
+  __device__ inline void load_full(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int m0, int n0,
+                                   int k0, int tid, bool vb) {
+    static_assert(CAN_VA, "full-slab loads need the vector A mapping");
+#pragma unroll
+    for (int v = 0; v < NA / VW; v++) {
+      const int vidx = tid + 256 * v;
+      const int i = !TA ? (vidx % (BM / VW)) * VW : vidx / (BK / VW);
+      const int kk = !TA ? vidx / (BM / VW) : (vidx % (BK / VW)) * VW;
+      const int gi = m0 + i, gk = k0 + kk;
+      const vecT x = *reinterpret_cast<const vecT *>(TA ? &A[gk + (long)gi * lda] : &A[gi + (long)gk * lda]);
+#pragma unroll
+      for (int e = 0; e < VW; e++) ra[v * VW + e] = x[e];
+    }
+    if (CAN_VB && vb) {
+#pragma unroll
+      for (int v = 0; v < NB / VW; v++) {
+        const int vidx = tid + 256 * v;
+        const int j = TB ? (vidx % (BN / VW)) * VW : vidx / (BK / VW);
+        const int kk = TB ? vidx / (BN / VW) : (vidx % (BK / VW)) * VW;
+        const int gj = n0 + j, gk = k0 + kk;
+        const vecT x = *reinterpret_cast<const vecT *>(TB ? &B[gj + (long)gk * ldb] : &B[gk + (long)gj * ldb]);
+#pragma unroll
+        for (int e = 0; e < VW; e++) rb[v * VW + e] = x[e];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NB; q++) {
+        const int idx = tid + 256 * q;
+        const int j = TB ? idx % BN : idx / BK, kk = TB ? idx / BN : idx % BK;
+        const int gj = n0 + j, gk = k0 + kk;
+        rb[q] = TB ? B[gj + (long)gk * ldb] : B[gk + (long)gj * ldb];
+      }
+    }
+  }
+
   // element (scalar) mapping: the split-precision staging reads ra / rb in it
   __device__ inline void load(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M, int N,
                               int m0, int n0, int k0, int ke, int tid, bool va = false, bool vb = false) {
@@ -265,7 +304,8 @@ struct KPair {
   int K1 = 0x7fffffff;
 };
 
-template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, typename VT, int BK = kBK>
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, typename T, typename VT, int BK = kBK, bool DEEP = false,
+          bool KCAT = false>
 __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb, int M,
                                      int N, int m0, int n0, int kb, int ke, T *As0, T *Bs0, VT (&acc)[BM / WM / 16][BN / WN / 16],
                                      KPair<T> kp = KPair<T>()) {
@@ -278,26 +318,18 @@ __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T 
   // 16-byte loads where the operands' alignment allows them
   constexpr int VW = L::VW;
   auto vec_ok = [](const T *p, long ld) { return ((reinterpret_cast<unsigned long>(p) & 15) == 0) && (ld % VW == 0); };
-  const bool va = vec_ok(A, lda) && (!kp.A2 || vec_ok(kp.A2, kp.lda2));
-  const bool vb = vec_ok(B, ldb) && (!kp.A2 || vec_ok(kp.B2, kp.ldb2));
+  const bool va = vec_ok(A, lda) && (!KCAT || vec_ok(kp.A2, kp.lda2));
+  const bool vb = vec_ok(B, ldb) && (!KCAT || vec_ok(kp.B2, kp.ldb2));
   // the K slab at k0 from the pair it lies in
   auto load_slab = [&](int k0) {
-    if (k0 < kp.K1)
+    if (!KCAT)
+      ld.load(A, lda, B, ldb, M, N, m0, n0, k0, ke, tid, va, vb);
+    else if (k0 < kp.K1)
       ld.load(A, lda, B, ldb, M, N, m0, n0, k0, min(ke, kp.K1), tid, va, vb);
     else
       ld.load(kp.A2, kp.lda2, kp.B2, kp.ldb2, M, N, m0, n0, k0 - kp.K1, ke - kp.K1, tid, va, vb);
   };
-  int nk = ke > kb ? (ke - kb + BK - 1) / BK : 0;
-  if (nk > 0) {
-    load_slab(kb);
-    ld.store(As0, Bs0, tid, va, vb);
-  }
-  __syncthreads();
-  for (int t = 0; t < nk; t++) {
-    const int cur = t & 1;
-    if (t + 1 < nk) load_slab(kb + (t + 1) * BK);
-    const T *As = As0 + cur * SA;
-    const T *Bs = Bs0 + cur * SB;
+  auto compute = [&](const T *As, const T *Bs) {
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ks++) {
       T a[MI], b[NI];
@@ -311,12 +343,74 @@ __device__ inline void tile_mainloop(const T *__restrict__ A, long lda, const T 
 #pragma unroll
         for (int ni = 0; ni < NI; ni++) acc[mi][ni] = mfma16(b[ni], a[mi], acc[mi][ni]);
     }
+  };
+  if constexpr (DEEP && !KCAT && L::CAN_VA) {
+    // Interior tiles of a single-pair product: slabs two ahead of the one in
+    // the MFMAs, in two register sets that alternate (the loop is unrolled
+    // by two), and a step body without branches around its loads, so the
+    // wait before an LDS store covers only the older set.  The HBM-bound
+    // long-K products need that much in flight.
+    const int nfull = (ke - kb) / BK;
+    if (va && m0 + BM <= M && n0 + BN <= N && nfull >= 2) {
+      L ld1;
+      ld.load_full(A, lda, B, ldb, m0, n0, kb, tid, vb);
+      ld1.load_full(A, lda, B, ldb, m0, n0, kb + BK, tid, vb);
+      ld.store(As0, Bs0, tid, true, vb);
+      __syncthreads();
+      // stage 0 holds slab t, ld1 carries slab t + 1
+      int t = 0;
+      for (; t + 3 < nfull; t += 2) {
+        ld.load_full(A, lda, B, ldb, m0, n0, kb + (t + 2) * BK, tid, vb);
+        compute(As0, Bs0);
+        ld1.store(As0 + SA, Bs0 + SB, tid, true, vb);
+        __syncthreads();
+        ld1.load_full(A, lda, B, ldb, m0, n0, kb + (t + 3) * BK, tid, vb);
+        compute(As0 + SA, Bs0 + SB);
+        ld.store(As0, Bs0, tid, true, vb);
+        __syncthreads();
+      }
+      const bool three = t + 2 < nfull;
+      if (three) ld.load_full(A, lda, B, ldb, m0, n0, kb + (t + 2) * BK, tid, vb);
+      compute(As0, Bs0);
+      ld1.store(As0 + SA, Bs0 + SB, tid, true, vb);
+      __syncthreads();
+      compute(As0 + SA, Bs0 + SB);
+      int done = t + 2;
+      if (three) {
+        ld.store(As0, Bs0, tid, true, vb);
+        __syncthreads();
+        compute(As0, Bs0);
+        done = t + 3;
+      }
+      // the partial slab at the end of the K range, through the checked path
+      const int k0 = kb + done * BK;
+      if (k0 < ke) {
+        __syncthreads();
+        ld.load(A, lda, B, ldb, M, N, m0, n0, k0, ke, tid, va, vb);
+        ld.store(As0, Bs0, tid, va, vb);
+        __syncthreads();
+        compute(As0, Bs0);
+      }
+      __syncthreads();
+      return;
+    }
+  }
+  int nk = ke > kb ? (ke - kb + BK - 1) / BK : 0;
+  if (nk > 0) {
+    load_slab(kb);
+    ld.store(As0, Bs0, tid, va, vb);
+  }
+  __syncthreads();
+  for (int t = 0; t < nk; t++) {
+    const int cur = t & 1;
+    if (t + 1 < nk) load_slab(kb + (t + 1) * BK);
+    compute(As0 + cur * SA, Bs0 + cur * SB);
     if (t + 1 < nk) ld.store(As0 + (cur ^ 1) * SA, Bs0 + (cur ^ 1) * SB, tid, va, vb);
     __syncthreads();
   }
 }
 
-template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK>
+template <bool TA, bool TB, int BM, int BN, int WM, int WN, int BK = kBK, bool KCAT = false>
 __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float alpha, const float *__restrict__ A,
                                                     long lda, long sA, const float *__restrict__ B, long ldb, long sB,
                                                     float beta, float *__restrict__ C, long ldc, long sC, int kchunk,
@@ -332,7 +426,7 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
   A += batch * sA;
   B += batch * sB;
   C += batch * sC;
-  if (kp.A2) {
+  if (KCAT) {
     kp.A2 += batch * sA2;
     kp.B2 += batch * sB2;
   }
@@ -344,7 +438,9 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
   for (int mi = 0; mi < MI; mi++)
 #pragma unroll
     for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
-  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK>(A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc, kp);
+  // the long-K A W products (A not transposed) take the two-slab pipeline
+  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK, !TA && BM == 256 && BN == 32, KCAT>(
+      A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc, kp);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -428,9 +524,14 @@ static int launch_sgemm(evd_ctx *ctx, long m, long n, long k, float alpha, const
     if (!P) return EVD_E_NOMEM;
   }
   dim3 grid((unsigned)((m + BM - 1) / BM), (unsigned)((n + BN - 1) / BN), (unsigned)(batch * ksplit));
-  sgemm_kernel<TA, TB, BM, BN, WM, WN, BK><<<grid, 256, 0, ctx->stream>>>(
-      (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch, kc.kp,
-      kc.sA2, kc.sB2);
+  if (kc.kp.A2)
+    sgemm_kernel<TA, TB, BM, BN, WM, WN, BK, true><<<grid, 256, 0, ctx->stream>>>(
+        (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch, kc.kp,
+        kc.sA2, kc.sB2);
+  else
+    sgemm_kernel<TA, TB, BM, BN, WM, WN, BK, false><<<grid, 256, 0, ctx->stream>>>(
+        (int)m, (int)n, (int)k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, (int)kchunk, ksplit, P, batch, kc.kp,
+        kc.sA2, kc.sB2);
   EVD_LAUNCHED(ctx);
   if (ksplit > 1) {
     long total = m * n * batch;

@@ -16,7 +16,12 @@ def _torch():
 
 
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
-@pytest.mark.parametrize("m,n,k", [(300, 200, 100), (4000, 32, 3000), (32, 32, 5000), (64, 500, 70), (1, 7, 3)])
+@pytest.mark.parametrize(
+    "m,n,k",
+    [(300, 200, 100), (4000, 32, 3000), (32, 32, 5000), (64, 500, 70), (1, 7, 3),
+     # long-K skinny products (256 x 32 tiles, K split): interior tiles on the
+     # two-slab pipeline, an edge tile, a partial last slab, an unaligned lda
+     (5000, 32, 4100), (2048, 32, 4096), (4999, 32, 4103)])
 def test_sgemm_matches_fp64(ctx, ta, tb, m, n, k):
     import evd4sc2025_amd as E
 
