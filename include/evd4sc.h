@@ -84,10 +84,13 @@ int evd_ctx_set_tuning(evd_ctx *ctx, int bc_sweeps, int bc_waves, int chase_grou
  * bulge-chase workgroup, at most 3 and what its LDS window holds are used;
  * default 1), "panel_rows" (256 or 512 rows per panel-QR workgroup, default
  * 512), "gemm_tile" (1: 128 x 128, 2: 64 x 64 output tiles of this
- * library's FP32 products; default by shape) and "own_gemm" (1: the large
+ * library's FP32 products; default by shape), "own_gemm" (1: the large
  * plain FP32 products run on this library's MFMA kernels; by default they go
- * to rocBLAS on the context stream, deterministic mode).  Returns -2 for an
- * unknown name, -3 for a value out of range.
+ * to rocBLAS on the context stream, deterministic mode) and "panel_exchange"
+ * (1: every panel on the column-exchange kernel; by default panels of up to
+ * 32 columns take the blocked Cholesky QR path, with the exchange kernel as
+ * its on-device fallback).  Returns -2 for an unknown name, -3 for a value
+ * out of range.
  * No counterpart in the reference (its tile shapes are compile-time). */
 int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value);
 int evd_ctx_set_precision(evd_ctx *ctx, int precision);
