@@ -242,13 +242,36 @@ __global__ __launch_bounds__(256) void cq_gram_kernel(int m, int nc, const T *__
   sum_slots(part + (long)ng * kQQ, ngroups, part + (long)(ng + ngroups) * kQQ, tid);
 }
 
+// 1 / sqrt(x) and 1 / x for x > 0 (finite) to FP64 rounding level: the
+// hardware approximation refined by two Newton steps (each doubles the
+// correct bits), a short dependent chain next to the IEEE square root and
+// division sequences.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-x, y, 1.0);  // 1 - x y
+    y = fma(y, e, y);
+  }
+  return y;
+}
+
 // Upper Cholesky factor of the 32 x 32 matrix G (LDS, pitch kLD) into R (LDS),
-// on one wave, right-looking: lane c keeps working column c in registers and
-// takes the pivot and row j of R from the other lanes' registers (readlane),
-// so a step's dependent chain is the pivot, its square root and reciprocal,
-// and one multiply-add of the next pivot column; lanes 32..63 repeat lanes
-// 0..31 and store nothing.  Returns false when a pivot is not above tol_rel
-// times the largest diagonal entry (or is not finite).
+// on one wave in left-looking (Crout) order: lane c keeps column c of R in
+// registers and reads the finished rows k < j back from LDS as broadcasts; the
+// pivot comes from lane j (readlane) and its reciprocal square root from
+// rsqrt_nr (lanes 32..63 repeat lanes 0..31 and store nothing).  Returns false
+// when a pivot is not above tol_rel times the largest diagonal entry (or is
+// not finite).
 __device__ __forceinline__ bool wave_cholesky(const double *G, double *R, int lane, double tol_rel) {
   const int c = lane & (kQ - 1);
   double dmax = G[c * kLD + c];
@@ -256,19 +279,24 @@ __device__ __forceinline__ bool wave_cholesky(const double *G, double *R, int la
   for (int o = 16; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
   const double tol = tol_rel * dmax;
   bool ok = dmax > 0.0 && dmax < 1e300;
-  double col[kQ];
-#pragma unroll
-  for (int i = 0; i < kQ; i++) col[i] = G[i * kLD + c];
+  double r[kQ];
 #pragma unroll
   for (int j = 0; j < kQ; j++) {
-    const double d = lane_value(col[j], j);  // working G[j][j]
-    if (!(d > tol)) ok = false;
-    const double rinv = 1.0 / (ok ? sqrt(d) : 1.0);
-    const double rjc = (c >= j) ? col[j] * rinv : 0.0;  // R[j][c]
-    if (lane < kQ) R[j * kLD + c] = rjc;
-    // trailing update G[i][c] -= R[j][i] R[j][c], i > j
+    double s0 = G[j * kLD + c], s1 = 0.0;  // two chains of the dot product
 #pragma unroll
-    for (int i = j + 1; i < kQ; i++) col[i] -= lane_value(rjc, i) * rjc;
+    for (int k = 0; k < j; k++) {
+      if (k & 1)
+        s1 -= R[k * kLD + j] * r[k];
+      else
+        s0 -= R[k * kLD + j] * r[k];
+    }
+    const double s = s0 + s1;
+    const double d = lane_value(s, j);  // G[j][j] - sum_k R[k][j]^2
+    if (!(d > tol)) ok = false;
+    const double y = ok ? rsqrt_nr(d) : 1.0;
+    r[j] = (c >= j) ? s * y : 0.0;
+    if (lane < kQ) R[j * kLD + c] = r[j];
+    lds_order();
   }
   return ok;
 }
@@ -345,6 +373,14 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
   gram += q * s_part;
   fac += (long)q * kFactorWords;
   A += q * s_in;
+  // the panel's top 32 x 32 block (read for Q_top below), loaded now so the
+  // loads overlap the factorisation
+  T atop[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int e = tid + 256 * r, i = e % kQ, k = e / kQ;
+    atop[r] = A[min(i, m - 1) + (long)min(k, nc - 1) * lda];
+  }
   __shared__ double G[kQ * kLD], R[kQ * kLD], X[kQ * kLD], Y[kQ * kLD], Z[kQ * kLD], U[kQ * kLD];
   __shared__ double sgn[kQ];
   __shared__ double red[4];
@@ -413,18 +449,10 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
   __syncthreads();
   // ---- Q_top = A_top R^-1 (rows past m and padded columns are zero): A_top
   // into X, the product into Z
-  {
-    T v[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int e = tid + 256 * r, i = e % kQ, k = e / kQ;
-      v[r] = A[min(i, m - 1) + (long)min(k, nc - 1) * lda];
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int e = tid + 256 * r, i = e % kQ, k = e / kQ;
-      X[i * kLD + k] = (k < nc && i < m) ? (double)v[r] : 0.0;
-    }
+  for (int r = 0; r < 4; r++) {
+    const int e = tid + 256 * r, i = e % kQ, k = e / kQ;
+    X[i * kLD + k] = (k < nc && i < m) ? (double)atop[r] : 0.0;
   }
   __syncthreads();
   block_matmul(X, G, Z, tid);
@@ -442,7 +470,7 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
       const double piv = lane_value(row[j], j);
       const double sj = (piv >= 0.0) ? -1.0 : 1.0;
       const double ujj = piv - sj;  // |ujj| >= 1
-      const double uinv = 1.0 / ujj;
+      const double uinv = rcp_nr(ujj);
       double u[kQ];  // row j of U' (wave-uniform)
 #pragma unroll
       for (int c = j + 1; c < kQ; c++) u[c] = lane_value(row[c], j);
@@ -500,23 +528,44 @@ __global__ __launch_bounds__(256) void cq_out_kernel(int m, int nc, const T *__r
                                                      long ldr, double *__restrict__ tau_out,
                                                      const int *__restrict__ state) {
   const int g = blockIdx.x, q = blockIdx.y, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (state[q] != kDone) return;
+  // every global load of the kernel is issued before the first wait: the
+  // state word, this group's rows, M and T, and (group 0) V1 and R_H
+  const int st = state[q];
   A += q * s_in;
   Y += q * s_y;
   W += q * s_w;
   if (Aout) Aout += q * s_out;
   fac += (long)q * kFactorWords;
   __shared__ double xs[kRows * kLD];
-  __shared__ double Ms[kQ * kLD], Ts[kQ * kLD];
+  __shared__ double Ms[kQ * kLD], Ts[kQ * kLD], V1s[kQ * kLD], RHs[kQ * kLD];
   const long r = (long)g * kRows + tid;
-  {
-    double a[kQ];
-    load_row(A, lda, m, nc, r, a);
+  double a[kQ], fv[4][4];
+  load_row(A, lda, m, nc, r, a);
 #pragma unroll
-    for (int c = 0; c < kQ; c++) xs[tid * kLD + c] = a[c];
+  for (int u = 0; u < 4; u++) {
+    fv[0][u] = fac[kF_M + tid + 256 * u];
+    fv[1][u] = fac[kF_T + tid + 256 * u];
   }
-  load_mat(fac + kF_M, Ms, tid);
-  load_mat(fac + kF_T, Ts, tid);
+  if (g == 0) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      fv[2][u] = fac[kF_V1 + tid + 256 * u];
+      fv[3][u] = fac[kF_RH + tid + 256 * u];
+    }
+  }
+  if (st != kDone) return;
+#pragma unroll
+  for (int c = 0; c < kQ; c++) xs[tid * kLD + c] = a[c];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int e = tid + 256 * u, o = (e / kQ) * kLD + e % kQ;
+    Ms[o] = fv[0][u];
+    Ts[o] = fv[1][u];
+    if (g == 0) {
+      V1s[o] = fv[2][u];
+      RHs[o] = fv[3][u];
+    }
+  }
   __syncthreads();
   f64x4 acc[4][2];
   wave_rows_times(xs, Ms, w, lane, acc);
@@ -524,7 +573,7 @@ __global__ __launch_bounds__(256) void cq_out_kernel(int m, int nc, const T *__r
   wave_store_rows(xs, w, lane, acc);  // V (wave-local rows)
   if (r < kQ) {
 #pragma unroll
-    for (int c = 0; c < kQ; c++) xs[tid * kLD + c] = fac[kF_V1 + r * kQ + c];
+    for (int c = 0; c < kQ; c++) xs[tid * kLD + c] = V1s[r * kLD + c];
   }
   lds_order();
   wave_rows_times(xs, Ts, w, lane, acc);  // W = V T
@@ -542,7 +591,7 @@ __global__ __launch_bounds__(256) void cq_out_kernel(int m, int nc, const T *__r
       if (c >= nc) continue;
       if (W) W[r + (long)c * ldw] = (T)xs[tid * kLD + c];
       if (r < kQ) {
-        const double rv = (r <= c) ? fac[kF_RH + r * kQ + c] : 0.0;
+        const double rv = (r <= c) ? RHs[r * kLD + c] : 0.0;
         if (Aout) Aout[r + (long)c * ldo] = (T)rv;
         if (R && r < nc) R[r + (long)c * ldr] = (T)rv;
       } else if (Aout) {
