@@ -7,6 +7,8 @@
 //   Z_p = A W_p - Y_q (Z_q^T W_p) - Z_q (Y_q^T W_p)      (lazy: A is the
 //         level-start trailing matrix, q = the level's earlier panels)
 //   Z_p -= 1/2 Y_p (W_p^T Z_p)
+// (W_p^T Z_p formed from W_p^T A W_p and the two small products, so the
+// three corrections are one product; see sy2sb)
 // so that (I - Y W^T) A (I - W Y^T) = A - Y Z^T - Z Y^T, and only the next
 // panel's b columns are updated inside the level.  At the level end the
 // trailing matrix gets one SYR2K with k = nb on MFMA, mirrored to full
@@ -70,25 +72,35 @@ int sy2sb(evd_ctx *ctx, long n, long b, long nb, float *A, long lda, float *W, l
       // Z_p = A[r0:, r0:] W_p  (A here is the level-start trailing matrix)
       EVD_TRY(sgemm(ctx, 'N', 'N', m, b, m, 1.f, A + r0 + r0 * lda, lda, Wp, ldw, 0.f, Zp, ldz));
       if (p > 0) {
-        // Z_p -= Y_q (Z_q^T W_p) + Z_q (Y_q^T W_p): the two small products in
-        // one launch (a batch of two over Z_q and Y_q, shaped as one product),
-        // the update as one K-concatenated product
-        const long pb = p * b;
+        // Z_p -= Y_q t_z + Z_q t_y + Y_p t3 / 2 with t_z = Z_q^T W_p,
+        // t_y = Y_q^T W_p and t3 = W_p^T Z_p (after the first two terms).
+        // [Z_q Z_p]^T W_p and [Y_q Y_p]^T W_p run as one launch (a batch of
+        // two, shaped as one product; Z_p and Y_p follow Z_q and Y_q in
+        // memory) and give t_z, t_y and X1 = Z_p^T W_p (Z_p = A W_p so far).
+        // t3 = W_p^T A' W_p is symmetric in exact arithmetic (A' the updated
+        // trailing matrix), so t3^T = X1 - t_y^T t_z - t_z^T t_y stands in for
+        // it: formed in place of X1 as a short K-concatenated product (K =
+        // 2 pb instead of a product over the m rows), [t_z; t3^T / 2] is then
+        // the right operand of Y_q t_z + Y_p t3 / 2, and the update is one
+        // K-concatenated product
+        const long pb = p * b, pbx = pb + b;
         const float *Yq = Y + r0 + c0 * ldy;
         const float *Zq = Z + r0;
-        float *tz = t, *ty = t + pb * b;
+        float *tz = t, *ty = t + pbx * b;
         if (ldy == ldz) {
           ShapeDiv pair(ctx, 2);
           const long dYZ = (long)(reinterpret_cast<intptr_t>(Yq) - reinterpret_cast<intptr_t>(Zq)) / (long)sizeof(float);
-          EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pb, 2, dYZ, 0, pb * b));
+          EVD_TRY(sgemm(ctx, 'T', 'N', pbx, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pbx, 2, dYZ, 0, pbx * b));
         } else {
-          EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pb));
-          EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, ty, pb));
+          EVD_TRY(sgemm(ctx, 'T', 'N', pbx, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pbx));
+          EVD_TRY(sgemm(ctx, 'T', 'N', pbx, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, ty, pbx));
         }
-        EVD_TRY(sgemm_kcat(ctx, 'N', 'N', m, b, pb, pb, -1.f, Yq, ldy, tz, pb, Zq, ldz, ty, pb, 1.f, Zp, ldz));
+        EVD_TRY(sgemm_kcat(ctx, 'T', 'N', b, b, pb, pb, -0.5f, ty, pbx, tz, pbx, tz, pbx, ty, pbx, 0.5f, tz + pb, pbx));
+        EVD_TRY(sgemm_kcat(ctx, 'N', 'N', m, b, pbx, pb, -1.f, Yq, ldy, tz, pbx, Zq, ldz, ty, pbx, 1.f, Zp, ldz));
+      } else {
+        EVD_TRY(sgemm(ctx, 'T', 'N', b, b, m, 1.f, Wp, ldw, Zp, ldz, 0.f, t3, b));
+        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, b, -0.5f, Yp, ldy, t3, b, 1.f, Zp, ldz));
       }
-      EVD_TRY(sgemm(ctx, 'T', 'N', b, b, m, 1.f, Wp, ldw, Zp, ldz, 0.f, t3, b));
-      EVD_TRY(sgemm(ctx, 'N', 'N', m, b, b, -0.5f, Yp, ldy, t3, b, 1.f, Zp, ldz));
       const long kk = (p + 1) * b;
       const float *Ya = Y + r0 + c0 * ldy;
       const float *Za = Z + r0;
@@ -160,14 +172,18 @@ int sy2sb_batched(evd_ctx *ctx, int count, long n, long b, long nb, float *A, lo
         const long pb = p * b;
         const float *Yq = Y + r0 + c0 * ldy;
         const float *Zq = Z + r0;
-        float *tz = t, *ty = t + pb * b;
-        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pb, count, sZ, sW, sT));
-        EVD_TRY(sgemm(ctx, 'T', 'N', pb, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, ty, pb, count, sY, sW, sT));
-        EVD_TRY(sgemm_kcat(ctx, 'N', 'N', m, b, pb, pb, -1.f, Yq, ldy, tz, pb, Zq, ldz, ty, pb, 1.f, Zp, ldz, count, sY,
-                           sT, sZ, sT, sZ));
+        const long pbx = pb + b;
+        float *tz = t, *ty = t + pbx * b;
+        EVD_TRY(sgemm(ctx, 'T', 'N', pbx, b, m, 1.f, Zq, ldz, Wp, ldw, 0.f, tz, pbx, count, sZ, sW, sT));
+        EVD_TRY(sgemm(ctx, 'T', 'N', pbx, b, m, 1.f, Yq, ldy, Wp, ldw, 0.f, ty, pbx, count, sY, sW, sT));
+        EVD_TRY(sgemm_kcat(ctx, 'T', 'N', b, b, pb, pb, -0.5f, ty, pbx, tz, pbx, tz, pbx, ty, pbx, 0.5f, tz + pb, pbx,
+                           count, sT, sT, sT, sT, sT));
+        EVD_TRY(sgemm_kcat(ctx, 'N', 'N', m, b, pbx, pb, -1.f, Yq, ldy, tz, pbx, Zq, ldz, ty, pbx, 1.f, Zp, ldz, count,
+                           sY, sT, sZ, sT, sZ));
+      } else {
+        EVD_TRY(sgemm(ctx, 'T', 'N', b, b, m, 1.f, Wp, ldw, Zp, ldz, 0.f, t3, b, count, sW, sZ, sT3));
+        EVD_TRY(sgemm(ctx, 'N', 'N', m, b, b, -0.5f, Yp, ldy, t3, b, 1.f, Zp, ldz, count, sY, sT3, sZ));
       }
-      EVD_TRY(sgemm(ctx, 'T', 'N', b, b, m, 1.f, Wp, ldw, Zp, ldz, 0.f, t3, b, count, sW, sZ, sT3));
-      EVD_TRY(sgemm(ctx, 'N', 'N', m, b, b, -0.5f, Yp, ldy, t3, b, 1.f, Zp, ldz, count, sY, sT3, sZ));
       const long kk = (p + 1) * b;
       const float *Ya = Y + r0 + c0 * ldy;
       const float *Za = Z + r0;
