@@ -438,8 +438,8 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
   for (int mi = 0; mi < MI; mi++)
 #pragma unroll
     for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // the long-K A W products (A not transposed) take the two-slab pipeline
-  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK, !TA && BM == 256 && BN == 32, KCAT>(
+  // the long-K skinny products take the two-slab pipeline
+  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK, BM == 256 && BN == 32, KCAT>(
       A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc, kp);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
