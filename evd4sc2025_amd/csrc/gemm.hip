@@ -1021,13 +1021,12 @@ static int sgemm_split_t(evd_ctx *ctx, bool TA, bool TB, long m, long n, long k,
                          long sA, const float *B, long ldb, long sB, float beta, float *C, long ldc, long sC,
                          long batch, bool *handled) {
   *handled = true;
-  // long-K tall-skinny products (A W with K = the trailing size, the long
-  // reductions of the ZY updates)
-  if (n <= 32 && m > 64 && k >= 1024 && !TB) {
-    if (!TA) return launch_sgemm_split<S, false, false, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
-    return launch_sgemm_split<S, true, false, 256, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
-  }
-  // large square-ish products (back transformation, WY merges)
+  // The long-K tall-skinny products (A W with K = the trailing size, the
+  // long reductions of the ZY updates) stay on the FP32 kernels: they are
+  // bound by HBM and latency, and the FP32 kernels' two-slab pipeline runs
+  // them faster than the split 16-bit staging (n = 16384 EVD: FP16 form
+  // 0.537 -> 0.532 s, BF16 form 0.694 -> 0.638 s).  Large square-ish
+  // products (back transformation, WY merges) are the split kernels' work.
   if (m > 64 && n > 64 && k >= 64) {
     if (!TA && !TB) return launch_sgemm_split<S, false, false, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     if (TA && !TB) return launch_sgemm_split<S, true, false, 128, 128, 2, 2>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
