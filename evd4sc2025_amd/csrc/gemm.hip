@@ -652,16 +652,17 @@ int sgemm(evd_ctx *ctx, char ta, char tb, long m, long n, long k, float alpha, c
 // C = alpha (op(A1) op(B1) + op(A2) op(B2)) + beta C, the two products (inner
 // dimensions k1 and k2, same transposition) summed in one K loop: one launch
 // and one K split instead of two.  Falls back to two products when k1 is not
-// a multiple of the K slab, for the split-precision modes and for products
-// that go to the library.
+// a multiple of the K slab, for shapes the split-precision modes run on 16-bit
+// MFMA (the large square-ish ones) and for products that go to the library.
 int sgemm_kcat(evd_ctx *ctx, char ta, char tb, long m, long n, long k1, long k2, float alpha, const float *A1,
                long lda1, const float *B1, long ldb1, const float *A2, long lda2, const float *B2, long ldb2, float beta,
                float *C, long ldc, long batch, long sA1, long sB1, long sA2, long sB2, long sC) {
   if (m <= 0 || n <= 0 || batch <= 0) return 0;
   const long k = k1 + k2;
   const bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
-  if (k1 <= 0 || k2 <= 0 || k1 % kBK != 0 || alpha == 0.f || ctx->precision != EVD_PREC_FP32 ||
-      library_gemm(ctx, m, n, k) || k >= (1L << 31)) {
+  const bool split_shape = ctx->precision != EVD_PREC_FP32 && m > 64 && n > 64 && k >= 64;
+  if (k1 <= 0 || k2 <= 0 || k1 % kBK != 0 || alpha == 0.f || split_shape ||
+      (ctx->precision == EVD_PREC_FP32 && library_gemm(ctx, m, n, k)) || k >= (1L << 31)) {
     EVD_TRY(sgemm(ctx, ta, tb, m, n, k1, alpha, A1, lda1, B1, ldb1, beta, C, ldc, batch, sA1, sB1, sC));
     return sgemm(ctx, ta, tb, m, n, k2, alpha, A2, lda2, B2, ldb2, 1.f, C, ldc, batch, sA2, sB2, sC);
   }
