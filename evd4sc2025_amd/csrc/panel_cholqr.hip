@@ -18,19 +18,20 @@
 // so that (I - V T V^T) [R_H; 0] = A.  The sign rule is LAPACK's (beta =
 // -sign(alpha) ||x||), so V, T and R_H are dgeqrf's reflectors.
 //
-// MI355X mapping: five stream-ordered launches instead of one exchange per
+// MI355X mapping: three stream-ordered launches instead of one exchange per
 // column.  The Gram matrices are 32 x 32 FP64 products over the panel's rows
 // on v_mfma_f64_16x16x4f64, per workgroup of 256 rows, summed in a fixed
-// order by the factor kernels (one workgroup per problem: Cholesky, triangular
-// inverses, the sign-chosen LU on one wave); the output kernel forms V, W and
-// [R; 0] row by row.  Same-size panels of several problems run as one launch
-// each (grid y = problem).
+// order inside the launch; the workgroup that forms a problem's sum factors it
+// (Cholesky, triangular inverses, the sign-chosen LU on one wave); the second
+// pass does the same when the first asked for it; the output kernel forms V,
+// W and [R; 0] row by row.  Same-size panels of several problems run as one
+// launch each (grid y = problem).
 //
 // A panel whose Gram matrix is not safely positive definite (rank deficient or
 // condition number above about 1e6: Cholesky QR would lose orthogonality) is
-// flagged on the device; the later launches of that problem exit at once and
-// the column-exchange kernel (panel_qr.hip), launched after them, factors it
-// (and exits at once for the problems that succeeded).
+// flagged on the device; the later launches skip that problem and the
+// column-exchange kernel (panel_qr.hip), launched after them, factors it (and
+// exits at once for the problems that succeeded).
 #include "common.hpp"
 
 namespace evd {
@@ -103,8 +104,9 @@ __device__ __forceinline__ void load_mat(const double *__restrict__ src, double 
 // dst = sum of the n 32 x 32 matrices src[0 .. n) (kQQ doubles apart), added
 // in index order; written by other workgroups of this launch, so read and
 // written with device-coherent accesses.  Call from all 256 threads.
-__device__ __forceinline__ void sum_slots(const double *src, int n, double *dst, int tid) {
-  double sum[4] = {0.0, 0.0, 0.0, 0.0};
+__device__ __forceinline__ void sum_slots(const double *src, int n, double *dst, int tid, double (&sum)[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) sum[r] = 0.0;
   for (int g0 = 0; g0 < n; g0 += 8) {
     double v[8][4];
 #pragma unroll
@@ -179,23 +181,20 @@ __device__ __forceinline__ void wave_store_rows(double *xs, int w, int lane, con
 // last group to finish sums those into the final slot (always in index
 // order, so the sum does not depend on arrival order).  part per problem:
 // [ng partials][ngroups level-2 sums][final]; cnt: ngroups + 1 counters per
-// problem, zero between launches.
+// problem, zero between launches.  Returns true in the workgroup that formed
+// the final sum, whose thread t then holds its elements t + 256 r in gv[r].
+// A, fac, part and cnt point at this problem's data; xs has kRows * kLD
+// doubles of LDS (the rows, then the waves' partial sums) and ri (MODE 1)
+// kQ * kLD.
 template <int MODE, typename T>
-__global__ __launch_bounds__(256) void cq_gram_kernel(int m, int nc, const T *__restrict__ A, long lda, long s_in,
-                                                      const double *__restrict__ fac, double *__restrict__ part,
-                                                      long s_part, unsigned *__restrict__ cnt,
-                                                      const int *__restrict__ state) {
-  const int g = blockIdx.x, q = blockIdx.y, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int ng = gridDim.x, ngroups = (ng + kGroup - 1) / kGroup;
-  if (MODE == 1 && state[q] != kNeedPass2) return;
-  A += q * s_in;
-  part += q * s_part;
-  cnt += q * (ngroups + 1);
-  __shared__ double xs[kRows * kLD];  // this group's rows (at the end: the four waves' partial sums)
-  __shared__ double ri[kQ * kLD];
+__device__ __forceinline__ bool gram_pass(int m, int nc, const T *__restrict__ A, long lda,
+                                          const double *__restrict__ fac, double *part, unsigned *cnt, int g, int ng,
+                                          double *xs, double *ri, double (&gv)[4]) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int ngroups = (ng + kGroup - 1) / kGroup;
   double a[kQ];
   load_row(A, lda, m, nc, (long)g * kRows + tid, a);
-  if (MODE == 1) load_mat(fac + (long)q * kFactorWords + kF_R1I, ri, tid);
+  if (MODE == 1) load_mat(fac + kF_R1I, ri, tid);
 #pragma unroll
   for (int c = 0; c < kQ; c++) xs[tid * kLD + c] = a[c];
   if (MODE == 1) {
@@ -236,10 +235,11 @@ __global__ __launch_bounds__(256) void cq_gram_kernel(int m, int nc, const T *__
     st_dev(&part[(long)g * kQQ + e], ((xs[e] + xs[kQQ + e]) + xs[2 * kQQ + e]) + xs[3 * kQQ + e]);
   }
   const int grp = g / kGroup, gsize = min(kGroup, ng - grp * kGroup);
-  if (!last_arrival(cnt + grp, (unsigned)gsize)) return;
-  sum_slots(part + (long)grp * kGroup * kQQ, gsize, part + (long)(ng + grp) * kQQ, tid);
-  if (!last_arrival(cnt + ngroups, (unsigned)ngroups)) return;
-  sum_slots(part + (long)ng * kQQ, ngroups, part + (long)(ng + ngroups) * kQQ, tid);
+  if (!last_arrival(cnt + grp, (unsigned)gsize)) return false;
+  sum_slots(part + (long)grp * kGroup * kQQ, gsize, part + (long)(ng + grp) * kQQ, tid, gv);
+  if (!last_arrival(cnt + ngroups, (unsigned)ngroups)) return false;
+  sum_slots(part + (long)ng * kQQ, ngroups, part + (long)(ng + ngroups) * kQQ, tid, gv);
+  return true;
 }
 
 // 1 / sqrt(x) and 1 / x for x > 0 (finite) to FP64 rounding level: the
@@ -357,45 +357,30 @@ __device__ __forceinline__ double block_sum(double v, double *red, int tid) {
   return ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// Factor kernel, one workgroup per problem.  MODE 0: G1 -> R1, R1^-1; when
+// Factorisation of one problem's panel from its Gram matrix, by the workgroup
+// that formed the sum.  MODE 0: G1 -> R1, R1^-1; when
 // ||R1||_F ||R1^-1||_F <= one_pass_max (a bound on the condition number of
 // the panel small enough that Q1 = A R1^-1 is orthonormal to well below the
 // output precision) the reconstruction follows at once with R = R1, else R1
 // and R1^-1 are stored for the second pass.  MODE 1: G2 -> R2, R = R2 R1.
 // Reconstruction: Q_top = A_top R^-1, sign-chosen LU of Q_top - S, T,
 // M = R^-1 U'^-1, V1, R_H = S R, tau.
+// gv: the Gram matrix, elements tid + 256 r; atop: the panel's top block,
+// elements (i, k) = (e % 32, e / 32) of e = tid + 256 r; fac, state: this
+// problem's; sm: 6 kQ * kLD + kQ + 4 doubles of LDS.
 template <int MODE, typename T>
-__global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const double *__restrict__ gram, long s_part,
-                                                        double *__restrict__ fac, const T *__restrict__ A, long lda,
-                                                        long s_in, int *__restrict__ state, double one_pass_max) {
-  const int q = blockIdx.y, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (MODE == 1 && state[q] != kNeedPass2) return;
-  gram += q * s_part;
-  fac += (long)q * kFactorWords;
-  A += q * s_in;
-  // the panel's top 32 x 32 block (read for Q_top below), loaded now so the
-  // loads overlap the factorisation
-  T atop[4];
+__device__ __forceinline__ void factor_body(int m, int nc, const double (&gv)[4], const T (&atop)[4],
+                                            double *__restrict__ fac, int *state, double one_pass_max, double *sm) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double *G = sm, *R = sm + kQ * kLD, *X = sm + 2 * kQ * kLD, *Y = sm + 3 * kQ * kLD, *Z = sm + 4 * kQ * kLD,
+         *U = sm + 5 * kQ * kLD, *sgn = sm + 6 * kQ * kLD, *red = sgn + kQ;
+  __shared__ int s_ok;
+  // padded columns (c >= nc) get an identity block so the factorisation
+  // proceeds
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int e = tid + 256 * r, i = e % kQ, k = e / kQ;
-    atop[r] = A[min(i, m - 1) + (long)min(k, nc - 1) * lda];
-  }
-  __shared__ double G[kQ * kLD], R[kQ * kLD], X[kQ * kLD], Y[kQ * kLD], Z[kQ * kLD], U[kQ * kLD];
-  __shared__ double sgn[kQ];
-  __shared__ double red[4];
-  __shared__ int s_ok;
-  {
-    // the panel's Gram matrix (summed by the Gram kernel); padded columns
-    // (c >= nc) get an identity block so the factorisation proceeds
-    double v[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) v[r] = gram[tid + 256 * r];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int e = tid + 256 * r, i = e / kQ, j = e % kQ;
-      G[i * kLD + j] = (i >= nc || j >= nc) ? ((i == j) ? 1.0 : 0.0) : v[r];
-    }
+    const int e = tid + 256 * r, i = e / kQ, j = e % kQ;
+    G[i * kLD + j] = (i >= nc || j >= nc) ? ((i == j) ? 1.0 : 0.0) : gv[r];
   }
   __syncthreads();
   if (w == 0) {
@@ -406,7 +391,7 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
   }
   __syncthreads();
   if (!s_ok) {
-    if (tid == 0) state[q] = kFallback;
+    if (tid == 0) *state = kFallback;
     return;
   }
   // ---- Y <- R (the triangular factor of the panel), G <- R^-1
@@ -428,7 +413,7 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
         fac[kF_R1 + e] = R[(e / kQ) * kLD + e % kQ];
         fac[kF_R1I + e] = X[(e / kQ) * kLD + e % kQ];
       }
-      if (tid == 0) state[q] = kNeedPass2;
+      if (tid == 0) *state = kNeedPass2;
       return;
     }
     for (int e = tid; e < kQQ; e += 256) {
@@ -514,8 +499,39 @@ __global__ __launch_bounds__(256) void cq_factor_kernel(int m, int nc, const dou
     fac[kF_T + e] = (i <= j) ? Z[i * kLD + j] : 0.0;
   }
   if (tid < kQ) fac[kF_TAU + tid] = Z[tid * kLD + tid];
-  if (tid == 0) state[q] = kDone;
+  if (tid == 0) *state = kDone;
 }
+
+// One Gram pass and, in the workgroup that forms a problem's Gram sum, its
+// factorisation.  MODE 1 (the second pass) runs only for the problems whose
+// first pass asked for it.  Grid (ng, count), 256 rows per workgroup.
+template <int MODE, typename T>
+__global__ __launch_bounds__(256) void cq_pass_kernel(int m, int nc, const T *__restrict__ A, long lda, long s_in,
+                                                      double *__restrict__ fac, double *__restrict__ part,
+                                                      long s_part, unsigned *__restrict__ cnt,
+                                                      int *__restrict__ state, double one_pass_max) {
+  const int g = blockIdx.x, q = blockIdx.y, ng = gridDim.x, tid = threadIdx.x;
+  const int ngroups = (ng + kGroup - 1) / kGroup;
+  if (MODE == 1 && state[q] != kNeedPass2) return;
+  A += q * s_in;
+  fac += (long)q * kFactorWords;
+  // the panel's top 32 x 32 block, for the factorisation (loaded by every
+  // workgroup: which one forms the sum is known only at the end)
+  T atop[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int e = tid + 256 * r, i = e % kQ, k = e / kQ;
+    atop[r] = A[min(i, m - 1) + (long)min(k, nc - 1) * lda];
+  }
+  // LDS: the Gram pass's rows (+ R1^-1), then the factorisation's matrices
+  __shared__ double sm[kRows * kLD + (MODE == 1 ? kQ * kLD : 0)];
+  double gv[4];
+  if (!gram_pass<MODE, T>(m, nc, A, lda, fac, part + q * s_part, cnt + q * (ngroups + 1), g, ng, sm,
+                          sm + kRows * kLD, gv))
+    return;
+  factor_body<MODE, T>(m, nc, gv, atop, fac, state + q, one_pass_max, sm);
+}
+
 
 // Outputs, 256 rows per workgroup: V = A M (rows < 32: V1), W = V T on MFMA
 // through LDS (each wave on its own 64 rows), then Y = V, W, [R_H; 0] (Aout),
@@ -615,7 +631,6 @@ int panel_cholqr(evd_ctx *ctx, int count, long m, long nc, const T *A, long lda,
   const int ng = (int)((m + kRows - 1) / kRows);  // row workgroups
   const int ngroups = (ng + kGroup - 1) / kGroup;
   const long s_part = (long)(ng + ngroups + 1) * kQQ;
-  const long fin = (long)(ng + ngroups) * kQQ;  // the summed Gram matrix
   // problems per launch: the reduction counters of one launch must fit
   const int per = std::max(1, std::min(count, kCqCountWords / (ngroups + 1)));
   if (ngroups + 1 > kCqCountWords) return set_error(ctx, EVD_E_UNSUPPORTED, "panel of %ld rows", m);
@@ -633,16 +648,12 @@ int panel_cholqr(evd_ctx *ctx, int count, long m, long nc, const T *A, long lda,
     const T *Aq = A + q0 * s_in;
     T *Yq = Y + q0 * s_y, *Wq = W + q0 * s_w, *Oq = Aout ? Aout + q0 * s_out : nullptr;
     int *st = bad + q0;
-    const dim3 rows_grid((unsigned)ng, (unsigned)cq), one((unsigned)1, (unsigned)cq);
-    cq_gram_kernel<0, T><<<rows_grid, 256, 0, ctx->stream>>>(mi, ni, Aq, lda, s_in, fac, part, s_part, cnt, st);
+    const dim3 rows_grid((unsigned)ng, (unsigned)cq);
+    cq_pass_kernel<0, T><<<rows_grid, 256, 0, ctx->stream>>>(mi, ni, Aq, lda, s_in, fac, part, s_part, cnt, st,
+                                                             one_pass_max);
     EVD_LAUNCHED(ctx);
-    cq_factor_kernel<0, T><<<one, 256, 0, ctx->stream>>>(mi, ni, part + fin, s_part, fac, Aq, lda, s_in, st,
-                                                         one_pass_max);
-    EVD_LAUNCHED(ctx);
-    cq_gram_kernel<1, T><<<rows_grid, 256, 0, ctx->stream>>>(mi, ni, Aq, lda, s_in, fac, part, s_part, cnt, st);
-    EVD_LAUNCHED(ctx);
-    cq_factor_kernel<1, T><<<one, 256, 0, ctx->stream>>>(mi, ni, part + fin, s_part, fac, Aq, lda, s_in, st,
-                                                         one_pass_max);
+    cq_pass_kernel<1, T><<<rows_grid, 256, 0, ctx->stream>>>(mi, ni, Aq, lda, s_in, fac, part, s_part, cnt, st,
+                                                             one_pass_max);
     EVD_LAUNCHED(ctx);
     cq_out_kernel<T><<<rows_grid, 256, 0, ctx->stream>>>(mi, ni, Aq, lda, s_in, fac, Yq, ldy, s_y, Wq, ldw, s_w, Oq,
                                                         ldo, s_out, R, ldr, tau_out, st);
