@@ -21,7 +21,9 @@ def _torch():
     [(300, 200, 100), (4000, 32, 3000), (32, 32, 5000), (64, 500, 70), (1, 7, 3),
      # long-K skinny products (256 x 32 tiles, K split): interior tiles on the
      # two-slab pipeline, an edge tile, a partial last slab, an unaligned lda
-     (5000, 32, 4100), (2048, 32, 4096), (4999, 32, 4103)])
+     (5000, 32, 4100), (2048, 32, 4096), (4999, 32, 4103),
+     # K splits of up to 8 summed inside the launch (64 x 64 and 64 x 32 tiles)
+     (96, 96, 1024), (700, 32, 900)])
 def test_sgemm_matches_fp64(ctx, ta, tb, m, n, k):
     import evd4sc2025_amd as E
 
