@@ -321,9 +321,10 @@ def batched_form_bench(E, torch, count: int = 512, seed: int = 1):
     """Batched form: `count` symmetric FP32 matrices, n log-uniform in
     [64, 2048] (seed 1), uniform[-1,1] entries symmetrised, half with
     eigenvectors: one evd_syevd_mixed_f32 call (size buckets, one batched
-    solve per bucket and jobz, bandwidth min(32, N / 4) per bucket order N)
-    against the same problems solved one at a time with evd_syevd_f32 (b = 32);
-    both with inputs resident in HBM."""
+    solve per bucket and jobz, bandwidth min(32, N / 4) per bucket order N),
+    the same call captured as one hipGraph (evd_graph_create_mixed, mean of 3
+    replays), and the same problems solved one at a time with evd_syevd_f32
+    (b = 32); all with inputs resident in HBM."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -335,7 +336,7 @@ def batched_form_bench(E, torch, count: int = 512, seed: int = 1):
         E.syevd_mixed(ctx, mats[:16], jobz[:16], b=0)  # warm-up: arena, plans
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        E.syevd_mixed(ctx, mats, jobz, b=0)
+        eager = E.syevd_mixed(ctx, mats, jobz, b=0)
         torch.cuda.synchronize()
         t_mixed = time.perf_counter() - t0
         t0 = time.perf_counter()
@@ -343,10 +344,27 @@ def batched_form_bench(E, torch, count: int = 512, seed: int = 1):
             E.syevd(ctx, M, b=32, nb=256 if M.shape[0] < 4096 else 1024, jobz=j)
         torch.cuda.synchronize()
         t_single = time.perf_counter() - t0
+        # the same call captured once as a hipGraph (every bucket's padding,
+        # batched solve and extraction) and replayed on the resident inputs
+        g = E.MixedSolveGraph(ctx, mats, jobz=jobz, b=0)
+        try:
+            g.replay()
+            reps = 3
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                g.launch()
+            g.wait()
+            t_graph = (time.perf_counter() - t0) / reps
+            same = all(torch.equal(r.w, w) for r, w in zip(eager, g.w)) and all(
+                r.Z is None or torch.equal(r.Z, Z) for r, Z in zip(eager, g.Z))
+        finally:
+            g.close()
     finally:
         ctx.close()
     return {"matrices": count, "sizes": f"log-uniform [64, 2048], seed {seed}, half jobz=V",
             "bucketed_s": t_mixed, "bucketed_solves_per_s": count / t_mixed,
+            "captured_s": t_graph, "captured_solves_per_s": count / t_graph,
+            "captured_equals_eager": bool(same),
             "one_at_a_time_s": t_single, "one_at_a_time_solves_per_s": count / t_single,
             "speedup": t_single / t_mixed}
 

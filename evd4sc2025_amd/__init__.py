@@ -24,7 +24,7 @@ __all__ = [
     "Context", "EvdError", "syevd", "my_ZY_ZY_SBR_Vector", "sy2sb", "panelQR", "hou_tsqr_panel", "tc_ozimmu_syr2k", "band_pack",
     "sb2st", "stedc", "computerWYFromSlide", "computerQFromWY", "my_BC_back_trans_v8_10_noBandU", "bc_back",
     "sbr_back", "checkOrthogonality", "backward_error", "sgemm", "dgemm", "gen_symmetric", "gen_spectrum",
-    "STAGES", "autotune", "syevd_batched", "syevd_mixed", "SolveGraph", "BatchedSolveGraph", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
+    "STAGES", "autotune", "syevd_batched", "syevd_mixed", "SolveGraph", "BatchedSolveGraph", "MixedSolveGraph", "tune_cache_put", "tune_cache_get", "tune_cache_clear", "tune_cache_save", "tune_cache_load",
 ]
 
 # GEMM precision of SBR and the back transformation (evd_ctx_set_precision)
@@ -527,6 +527,54 @@ class BatchedSolveGraph(SolveGraph):
         self.launch()
         self.wait()
         return self.w, self.Z
+
+
+class MixedSolveGraph(SolveGraph):
+    """A mixed-size workload (evd_syevd_mixed_f32) captured as one hipGraph on
+    fixed buffers: every bucket's padding, batched solve and extraction.
+    `A[q]` holds a column-major copy of mats[q], `w[q]` (float64) and `Z[q]`
+    (None for jobz "N") its results.  Creating the graph solves `mats` once
+    eagerly.  `replay()` solves the current contents of A (left unchanged for
+    orders up to 4096); `solve(mats)` copies matrices of the same orders in
+    first.  Results are bitwise those of `syevd_mixed` on the same inputs."""
+
+    def __init__(self, ctx: Context, mats, jobz="V", b: int = 0):
+        torch = _torch()
+        self.ctx = ctx
+        count = len(mats)
+        self.jobz = [jobz] * count if isinstance(jobz, str) else list(jobz)
+        self.orders = [M.shape[0] for M in mats]
+        self.A = [colmajor(n, n, device=M.device) for n, M in zip(self.orders, mats)]
+        for A, M in zip(self.A, mats):
+            A.copy_(M)
+        self.w = [torch.empty(n, dtype=torch.float64, device=M.device) for n, M in zip(self.orders, mats)]
+        self.Z = [colmajor(n, n, device=M.device) if j == "V" else None
+                  for n, M, j in zip(self.orders, mats, self.jobz)]
+        L = C.c_long * count
+        Pv = C.c_void_p * count
+        n = L(*self.orders)
+        lda = L(*[_cm(A)[1] for A in self.A])
+        ldz = L(*[(_cm(Z)[1] if Z is not None else 1) for Z in self.Z])
+        pa = Pv(*[_cm(A)[0] for A in self.A])
+        pw = Pv(*[w.data_ptr() for w in self.w])
+        pz = Pv(*[(_cm(Z)[0] if Z is not None else 0) for Z in self.Z])
+        h = C.c_void_p()
+        ctx.call("evd_graph_create_mixed", count, n, pa, lda, pw, pz, ldz, "".join(self.jobz).encode(), b,
+                 C.byref(h))
+        self.h = h
+        _LIVE["graph"].add(self)
+
+    def replay(self):
+        self.launch()
+        self.wait()
+        return [EigResult(w, Z, {}) for w, Z in zip(self.w, self.Z)]
+
+    def solve(self, mats):
+        if [M.shape[0] for M in mats] != self.orders:
+            raise ValueError("graph was captured for other orders")
+        for A, M in zip(self.A, mats):
+            A.copy_(M)
+        return self.replay()
 
 
 def syevd_workspace(n: int, b: int, nb: int, jobz: str = "V") -> int:

@@ -63,6 +63,8 @@ SIGNATURES = {
                                  C.POINTER(_vp), C.POINTER(_l), C.c_char_p, _l]),
     "evd_graph_create": (_i, [_vp, _ch, _l, _l, _l, _vp, _l, _vp, _vp, _l, C.POINTER(_vp)]),
     "evd_graph_create_batched": (_i, [_vp, _ch, _i, _l, _l, _l, _vp, _l, _l, _vp, _l, _vp, _l, _l, C.POINTER(_vp)]),
+    "evd_graph_create_mixed": (_i, [_vp, _i, C.POINTER(_l), C.POINTER(_vp), C.POINTER(_l), C.POINTER(_vp),
+                                    C.POINTER(_vp), C.POINTER(_l), C.c_char_p, _l, C.POINTER(_vp)]),
     "evd_graph_launch": (_i, [_vp]),
     "evd_graph_wait": (_i, [_vp]),
     "evd_graph_destroy": (_i, [_vp]),

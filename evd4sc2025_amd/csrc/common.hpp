@@ -224,6 +224,28 @@ int syevd_call(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A, long 
 int syevd_batched_call(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda, long a_stride,
                        double *w, long w_stride, float *Z, long ldz, long z_stride);
 
+// evd_syevd_mixed_f32's arguments (host arrays of per-problem values)
+struct MixedArgs {
+  int count;
+  const long *n;
+  float *const *A;
+  const long *lda;
+  double *const *w;
+  float *const *Z;
+  const long *ldz;
+  const char *jobz;
+  long b;
+};
+// mixed.hip: argument checks; bytes of the per-group device descriptors and
+// their upload (synchronises the stream); arena bytes beyond the descriptors;
+// the solve of every group over uploaded descriptors (enqueues only, so it can
+// be captured)
+int mixed_check(evd_ctx *ctx, const MixedArgs &a);
+size_t mixed_desc_bytes(const MixedArgs &a);
+int mixed_upload(evd_ctx *ctx, const MixedArgs &a, char *dev);
+size_t mixed_stage_bytes(const evd_ctx *ctx, const MixedArgs &a);
+int mixed_enqueue(evd_ctx *ctx, const MixedArgs &a, const char *dev);
+
 }  // namespace evd
 
 // ---------------------------------------------------------------------------

@@ -12,6 +12,7 @@ struct evd_graph {
   evd_ctx *ctx = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  char *desc = nullptr;  // device descriptors the graph reads (mixed-size graphs)
 };
 
 using namespace evd;
@@ -95,6 +96,34 @@ extern "C" int evd_graph_create_batched(evd_ctx *ctx, char jobz, int count, long
       out);
 }
 
+extern "C" int evd_graph_create_mixed(evd_ctx *ctx, int count, const long *n, float *const *A, const long *lda,
+                                      double *const *w, float *const *Z, const long *ldz, const char *jobz, long b,
+                                      evd_graph **out) {
+  if (!ctx) return -1;
+  EVD_ARG(ctx, out != nullptr, 11);
+  EVD_ARG(ctx, count >= 1, 2);
+  *out = nullptr;
+  // eager solve: checks the arguments, sizes the arena, builds every
+  // bucket's merge plan
+  EVD_TRY(evd_syevd_mixed_f32(ctx, count, n, A, lda, w, Z, ldz, jobz, b));
+  const MixedArgs a{count, n, A, lda, w, Z, ldz, jobz, b};
+  // the descriptors live as long as the graph, outside the arena
+  const size_t bytes = mixed_desc_bytes(a);
+  char *desc = nullptr;
+  if (bytes) {
+    hipError_t e = hipMalloc(&desc, bytes);
+    if (e != hipSuccess) return set_error(ctx, EVD_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  int s = mixed_upload(ctx, a, desc);
+  if (s == 0) s = capture(ctx, [&] { return mixed_enqueue(ctx, a, desc); }, out);
+  if (s != 0) {
+    if (desc) hipFree(desc);
+    return s;
+  }
+  (*out)->desc = desc;
+  return 0;
+}
+
 extern "C" int evd_graph_launch(evd_graph *g) {
   if (!g) return -1;
   EVD_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
@@ -111,6 +140,7 @@ extern "C" int evd_graph_destroy(evd_graph *g) {
   hipStreamSynchronize(g->ctx->stream);
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);
+  if (g->desc) hipFree(g->desc);
   g->ctx->graphs--;
   delete g;
   return 0;

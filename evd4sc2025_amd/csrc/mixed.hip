@@ -98,7 +98,159 @@ struct Group {
   std::vector<int> members;
 };
 
+// Groups of (bucket order, eigenvector flag) in first-appearance order, and
+// the problems of orders above the largest bucket (solved alone, unpadded).
+void plan_groups(const MixedArgs &a, std::vector<Group> &groups, std::vector<int> &alone) {
+  for (int q = 0; q < a.count; q++) {
+    const char jz = (a.jobz[q] == 'V' || a.jobz[q] == 'v') ? 'V' : 'N';
+    if (a.n[q] > kMaxBucket) {
+      alone.push_back(q);
+      continue;
+    }
+    const long N = bucket_order(std::max<long>(a.n[q], a.b > 0 ? 2 * a.b : 64));
+    auto it = std::find_if(groups.begin(), groups.end(), [&](const Group &g) { return g.N == N && g.jobz == jz; });
+    if (it == groups.end()) {
+      groups.push_back(Group{N, jz, {}});
+      it = groups.end() - 1;
+    }
+    it->members.push_back(q);
+  }
+}
+
+long group_band(const MixedArgs &a, const Group &g) { return a.b > 0 ? a.b : std::min<long>(32, g.N / 4); }
+long group_nb(long bg) { return bg * std::max<long>(1, 1024 / bg); }
+
+// Device descriptors of one group: six arrays of c 8-byte entries (n, lda,
+// ldz, A, Z, w), each 256-byte aligned, back to back.
+size_t group_desc_bytes(int c) { return 6 * align256((size_t)c * 8); }
+
+// Arena bytes one group's solve takes beyond the descriptors.
+size_t group_stage_bytes(const evd_ctx *ctx, const MixedArgs &a, const Group &g) {
+  const int c = (int)g.members.size();
+  const long N = g.N, bg = group_band(a, g);
+  return align256((size_t)c * sizeof(double)) + align256((size_t)c * N * N * sizeof(float)) * (g.jobz == 'V' ? 2 : 1) +
+         align256((size_t)c * N * sizeof(double)) + syevd_batched_call_workspace(ctx, c, N, bg, group_nb(bg), g.jobz);
+}
+
+// One group: row-sum norms, padded copies, one batched solve, the leading
+// blocks back into the callers' buffers.  `desc` is the group's descriptor
+// block on the device.  Enqueues only (no host round trip).
+int solve_group(evd_ctx *ctx, const MixedArgs &a, const Group &g, const char *desc) {
+  const int c = (int)g.members.size();
+  const long N = g.N, bg = group_band(a, g);
+  const bool vec = g.jobz == 'V';
+  const size_t s = align256((size_t)c * 8);
+  const long *dn = reinterpret_cast<const long *>(desc), *dlda = reinterpret_cast<const long *>(desc + s),
+             *dldz = reinterpret_cast<const long *>(desc + 2 * s);
+  float *const *dA = reinterpret_cast<float *const *>(desc + 3 * s);
+  float *const *dZ = reinterpret_cast<float *const *>(desc + 4 * s);
+  double *const *dw = reinterpret_cast<double *const *>(desc + 5 * s);
+  EVD_TRY(arena_ensure(ctx, group_stage_bytes(ctx, a, g)));
+  ArenaFrame frame(ctx);
+  double *norm = arena_alloc_n<double>(ctx, c);
+  float *P = arena_alloc_n<float>(ctx, (size_t)c * N * N);
+  float *ZP = vec ? arena_alloc_n<float>(ctx, (size_t)c * N * N) : nullptr;
+  double *wP = arena_alloc_n<double>(ctx, (size_t)c * N);
+  if (!norm || !P || (vec && !ZP) || !wP) return EVD_E_NOMEM;
+  inf_norm_kernel<<<c, 256, 0, ctx->stream>>>(dn, dA, dlda, norm);
+  EVD_LAUNCHED(ctx);
+  const dim3 grid((unsigned)((N + 255) / 256), (unsigned)N, (unsigned)c);
+  pad_kernel<<<grid, 256, 0, ctx->stream>>>(N, dn, dA, dlda, norm, P);
+  EVD_LAUNCHED(ctx);
+  // the scaling of the input screen follows A's entries, not the pads: the
+  // split-FP16 form scales each problem to max |a_ij| in [0.5, 1), and a
+  // pad-driven scale would push A's entries toward FP16 subnormals (pads
+  // <= 2.5 ||A||_inf <= 2.5 n max |a_ij| stay inside the FP16 range, and
+  // the reduction never multiplies them with anything but exact zeros)
+  std::vector<long> rows(c);
+  for (int i = 0; i < c; i++) rows[i] = a.n[g.members[i]];
+  struct AmaxRows {
+    evd_ctx *c;
+    AmaxRows(evd_ctx *c_, const long *r) : c(c_) { c->amax_rows = r; }
+    ~AmaxRows() { c->amax_rows = nullptr; }
+  } amax_rows(ctx, rows.data());
+  EVD_TRY(syevd_batched_call(ctx, g.jobz, c, N, bg, group_nb(bg), P, N, N * N, wP, N, ZP, N, N * N));
+  unpad_kernel<<<grid, 256, 0, ctx->stream>>>(N, dn, wP, ZP, dw, dZ, dldz);
+  EVD_LAUNCHED(ctx);
+  return 0;
+}
+
 }  // namespace
+
+size_t mixed_desc_bytes(const MixedArgs &a) {
+  std::vector<Group> groups;
+  std::vector<int> alone;
+  plan_groups(a, groups, alone);
+  size_t bytes = 0;
+  for (const Group &g : groups) bytes += group_desc_bytes((int)g.members.size());
+  return bytes;
+}
+
+int mixed_upload(evd_ctx *ctx, const MixedArgs &a, char *dev) {
+  std::vector<Group> groups;
+  std::vector<int> alone;
+  plan_groups(a, groups, alone);
+  std::vector<char> host(mixed_desc_bytes(a));
+  if (host.empty()) return 0;
+  size_t off = 0;
+  for (const Group &g : groups) {
+    const int c = (int)g.members.size();
+    const size_t s = align256((size_t)c * 8);
+    const bool vec = g.jobz == 'V';
+    for (int i = 0; i < c; i++) {
+      const int q = g.members[i];
+      const long ldz = vec ? a.ldz[q] : 1;
+      float *Zq = vec ? a.Z[q] : nullptr;
+      std::memcpy(&host[off + i * 8], &a.n[q], 8);
+      std::memcpy(&host[off + s + i * 8], &a.lda[q], 8);
+      std::memcpy(&host[off + 2 * s + i * 8], &ldz, 8);
+      std::memcpy(&host[off + 3 * s + i * 8], &a.A[q], 8);
+      std::memcpy(&host[off + 4 * s + i * 8], &Zq, 8);
+      std::memcpy(&host[off + 5 * s + i * 8], &a.w[q], 8);
+    }
+    off += group_desc_bytes(c);
+  }
+  EVD_HIP(ctx, hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+  EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the host block is released on return
+  return 0;
+}
+
+size_t mixed_stage_bytes(const evd_ctx *ctx, const MixedArgs &a) {
+  std::vector<Group> groups;
+  std::vector<int> alone;
+  plan_groups(a, groups, alone);
+  size_t need = 0;
+  for (const Group &g : groups) need = std::max(need, group_stage_bytes(ctx, a, g));
+  for (int q : alone) need = std::max(need, syevd_call_workspace(ctx, a.n[q], a.b > 0 ? a.b : 32, 0, a.jobz[q]));
+  return need;
+}
+
+int mixed_enqueue(evd_ctx *ctx, const MixedArgs &a, const char *dev) {
+  std::vector<Group> groups;
+  std::vector<int> alone;
+  plan_groups(a, groups, alone);
+  size_t off = 0;
+  for (const Group &g : groups) {
+    EVD_TRY(solve_group(ctx, a, g, dev + off));
+    off += group_desc_bytes((int)g.members.size());
+  }
+  for (int q : alone)
+    EVD_TRY(syevd_call(ctx, a.jobz[q], a.n[q], a.b > 0 ? a.b : 32, 0, a.A[q], a.lda[q], a.w[q], a.Z[q], a.ldz[q]));
+  return 0;
+}
+
+int mixed_check(evd_ctx *ctx, const MixedArgs &a) {
+  EVD_ARG(ctx, a.count >= 0, 2);
+  EVD_ARG(ctx, a.count == 0 || (a.n && a.A && a.lda && a.w && a.Z && a.ldz && a.jobz), 3);
+  EVD_ARG(ctx, a.b == 0 || a.b == 32 || a.b == 64, 10);  // 0: b = min(32, N / 4) for each bucket order N
+  for (int q = 0; q < a.count; q++) {
+    const bool vec = (a.jobz[q] == 'V' || a.jobz[q] == 'v');
+    if (!(vec || a.jobz[q] == 'N' || a.jobz[q] == 'n')) return set_error(ctx, -9, "jobz[%d] invalid", q);
+    if (a.n[q] < 1 || a.lda[q] < a.n[q] || !a.A[q] || !a.w[q] || (vec && (!a.Z[q] || a.ldz[q] < a.n[q])))
+      return set_error(ctx, -3, "problem %d: invalid order, buffer or leading dimension", q);
+  }
+  return 0;
+}
 
 }  // namespace evd
 
@@ -107,94 +259,15 @@ using namespace evd;
 extern "C" int evd_syevd_mixed_f32(evd_ctx *ctx, int count, const long *n, float *const *A, const long *lda,
                                    double *const *w, float *const *Z, const long *ldz, const char *jobz, long b) {
   if (!ctx) return -1;
-  EVD_ARG(ctx, count >= 0, 2);
-  EVD_ARG(ctx, count == 0 || (n && A && lda && w && Z && ldz && jobz), 3);
-  EVD_ARG(ctx, b == 0 || b == 32 || b == 64, 10);  // 0: b = min(32, N / 4) for each bucket order N
+  const MixedArgs a{count, n, A, lda, w, Z, ldz, jobz, b};
+  EVD_TRY(mixed_check(ctx, a));
   if (ctx->arena_top != 0) return set_error(ctx, EVD_E_UNSUPPORTED, "evd_syevd_mixed_f32 cannot nest");
-  for (int q = 0; q < count; q++) {
-    const bool vec = (jobz[q] == 'V' || jobz[q] == 'v');
-    if (!(vec || jobz[q] == 'N' || jobz[q] == 'n')) return set_error(ctx, -9, "jobz[%d] invalid", q);
-    if (n[q] < 1 || lda[q] < n[q] || !A[q] || !w[q] || (vec && (!Z[q] || ldz[q] < n[q])))
-      return set_error(ctx, -3, "problem %d: invalid order, buffer or leading dimension", q);
-  }
-  // groups: (bucket order, eigenvector flag); large orders alone
-  std::vector<Group> groups;
-  std::vector<int> alone;
-  for (int q = 0; q < count; q++) {
-    const char jz = (jobz[q] == 'V' || jobz[q] == 'v') ? 'V' : 'N';
-    if (n[q] > kMaxBucket) {
-      alone.push_back(q);
-      continue;
-    }
-    const long N = bucket_order(std::max<long>(n[q], b > 0 ? 2 * b : 64));
-    auto it = std::find_if(groups.begin(), groups.end(), [&](const Group &g) { return g.N == N && g.jobz == jz; });
-    if (it == groups.end()) {
-      groups.push_back(Group{N, jz, {}});
-      it = groups.end() - 1;
-    }
-    it->members.push_back(q);
-  }
-  for (const Group &g : groups) {
-    const int c = (int)g.members.size();
-    const long N = g.N;
-    const bool vec = g.jobz == 'V';
-    const long bg = b > 0 ? b : std::min<long>(32, N / 4);  // the bucket's bandwidth
-    const long nb = bg * std::max<long>(1, 1024 / bg);
-    // staging: per-problem descriptors, norms, padded matrices and results
-    const size_t desc = align256((size_t)c * sizeof(long)) * 3 + align256((size_t)c * sizeof(void *)) * 3 +
-                        align256((size_t)c * sizeof(double));
-    const size_t stage = desc + align256((size_t)c * N * N * sizeof(float)) * (vec ? 2 : 1) +
-                         align256((size_t)c * N * sizeof(double));
-    EVD_TRY(arena_ensure(ctx, stage + syevd_batched_workspace(c, N, bg, nb, g.jobz, ctx->precision)));
-    ArenaFrame frame(ctx);
-    long *dn = arena_alloc_n<long>(ctx, c), *dlda = arena_alloc_n<long>(ctx, c), *dldz = arena_alloc_n<long>(ctx, c);
-    float **dA = arena_alloc_n<float *>(ctx, c), **dZ = arena_alloc_n<float *>(ctx, c);
-    double **dw = arena_alloc_n<double *>(ctx, c);
-    double *norm = arena_alloc_n<double>(ctx, c);
-    float *P = arena_alloc_n<float>(ctx, (size_t)c * N * N);
-    float *ZP = vec ? arena_alloc_n<float>(ctx, (size_t)c * N * N) : nullptr;
-    double *wP = arena_alloc_n<double>(ctx, (size_t)c * N);
-    if (!dn || !dlda || !dldz || !dA || !dZ || !dw || !norm || !P || (vec && !ZP) || !wP) return EVD_E_NOMEM;
-    // descriptors through one pinned-free path: a host staging vector copied
-    // with the stream (synchronised below before it is released)
-    std::vector<long> hn(c), hlda(c), hldz(c);
-    std::vector<float *> hA(c), hZ(c);
-    std::vector<double *> hw(c);
-    for (int i = 0; i < c; i++) {
-      const int q = g.members[i];
-      hn[i] = n[q];
-      hlda[i] = lda[q];
-      hldz[i] = vec ? ldz[q] : 1;
-      hA[i] = A[q];
-      hZ[i] = vec ? Z[q] : nullptr;
-      hw[i] = w[q];
-    }
-    EVD_HIP(ctx, hipMemcpyAsync(dn, hn.data(), sizeof(long) * c, hipMemcpyHostToDevice, ctx->stream));
-    EVD_HIP(ctx, hipMemcpyAsync(dlda, hlda.data(), sizeof(long) * c, hipMemcpyHostToDevice, ctx->stream));
-    EVD_HIP(ctx, hipMemcpyAsync(dldz, hldz.data(), sizeof(long) * c, hipMemcpyHostToDevice, ctx->stream));
-    EVD_HIP(ctx, hipMemcpyAsync(dA, hA.data(), sizeof(float *) * c, hipMemcpyHostToDevice, ctx->stream));
-    EVD_HIP(ctx, hipMemcpyAsync(dZ, hZ.data(), sizeof(float *) * c, hipMemcpyHostToDevice, ctx->stream));
-    EVD_HIP(ctx, hipMemcpyAsync(dw, hw.data(), sizeof(double *) * c, hipMemcpyHostToDevice, ctx->stream));
-    EVD_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope with this group
-    inf_norm_kernel<<<c, 256, 0, ctx->stream>>>(dn, dA, dlda, norm);
-    EVD_LAUNCHED(ctx);
-    const dim3 grid((unsigned)((N + 255) / 256), (unsigned)N, (unsigned)c);
-    pad_kernel<<<grid, 256, 0, ctx->stream>>>(N, dn, dA, dlda, norm, P);
-    EVD_LAUNCHED(ctx);
-    // the scaling of the input screen follows A's entries, not the pads: the
-    // split-FP16 form scales each problem to max |a_ij| in [0.5, 1), and a
-    // pad-driven scale would push A's entries toward FP16 subnormals (pads
-    // <= 2.5 ||A||_inf <= 2.5 n max |a_ij| stay inside the FP16 range, and
-    // the reduction never multiplies them with anything but exact zeros)
-    struct AmaxRows {
-      evd_ctx *c;
-      AmaxRows(evd_ctx *c_, const long *rows) : c(c_) { c->amax_rows = rows; }
-      ~AmaxRows() { c->amax_rows = nullptr; }
-    } amax_rows(ctx, hn.data());
-    EVD_TRY(syevd_batched_call(ctx, g.jobz, c, N, bg, nb, P, N, N * N, wP, N, ZP, N, N * N));
-    unpad_kernel<<<grid, 256, 0, ctx->stream>>>(N, dn, wP, ZP, dw, dZ, dldz);
-    EVD_LAUNCHED(ctx);
-  }
-  for (int q : alone) EVD_TRY(evd_syevd_f32(ctx, jobz[q], n[q], b > 0 ? b : 32, 0, A[q], lda[q], w[q], Z[q], ldz[q]));
+  const size_t desc = mixed_desc_bytes(a);
+  EVD_TRY(arena_ensure(ctx, align256(desc) + mixed_stage_bytes(ctx, a)));
+  ArenaFrame frame(ctx);
+  char *dev = arena_alloc_n<char>(ctx, desc);
+  if (desc && !dev) return EVD_E_NOMEM;
+  EVD_TRY(mixed_upload(ctx, a, dev));
+  EVD_TRY(mixed_enqueue(ctx, a, dev));
   return check_device_error(ctx, "evd_syevd_mixed_f32");
 }

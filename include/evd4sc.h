@@ -284,6 +284,14 @@ int evd_graph_create(evd_ctx *ctx, char jobz, long n, long b, long nb, float *A,
 int evd_graph_create_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, float *A, long lda,
                              long a_stride, double *w, long w_stride, float *Z, long ldz, long z_stride,
                              evd_graph **out);
+/* The same for a mixed-size workload (evd_syevd_mixed_f32 on fixed buffers):
+ * the whole call, every bucket's padding, batched solve and extraction, as one
+ * graph.  Replays read the current contents of A[q]; problems of order above
+ * 4096 are solved in place (A[q] consumed), so refill those before a replay.
+ * Results of a replay equal evd_syevd_mixed_f32 on the same inputs bit for
+ * bit.  The graph holds 48 bytes of device descriptors per problem. */
+int evd_graph_create_mixed(evd_ctx *ctx, int count, const long *n, float *const *A, const long *lda, double *const *w,
+                           float *const *Z, const long *ldz, const char *jobz, long b, evd_graph **out);
 int evd_graph_launch(evd_graph *g);
 int evd_graph_wait(evd_graph *g);
 int evd_graph_destroy(evd_graph *g);

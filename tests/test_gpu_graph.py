@@ -123,3 +123,51 @@ def test_batched_graph_replays_equal_eager(own_ctx, count, n, jobz):
                 assert np.array_equal(to_host(Z.contiguous()), eager[s][1]), f"replay {rep} (set {s}): vectors differ"
     finally:
         g.close()
+
+
+def test_mixed_graph_replays_equal_eager(own_ctx):
+    """evd_graph_create_mixed: the whole mixed-size call (five buckets, both
+    eigenvector flags, bandwidth min(32, N / 4)) as one graph.  Every replay
+    equals evd_syevd_mixed_f32 on the same inputs bit for bit, with eager
+    solves on the same context between replays; a replay without new inputs
+    solves A again (A is not modified)."""
+    import evd4sc2025_amd as E
+
+    torch = pytest.importorskip("torch")
+    orders = [70, 300, 64, 129, 700, 96, 300, 1000, 70, 150]
+    jobz = ["V" if q % 3 else "N" for q in range(len(orders))]
+    print(f"mixed graph case orders={orders}", flush=True)
+    sets = [[to_dev(O.gen_symmetric(n, 3000 * s + q, 0)) for q, n in enumerate(orders)] for s in range(3)]
+    g = E.MixedSolveGraph(own_ctx, sets[0], jobz=jobz, b=0)
+    try:
+        ref_ctx = E.Context(0)
+        try:
+            eager = []
+            for mats in sets:
+                res = E.syevd_mixed(ref_ctx, mats, jobz, b=0)
+                torch.cuda.synchronize()
+                eager.append([(to_host(r.w), to_host(r.Z) if r.Z is not None else None) for r in res])
+        finally:
+            ref_ctx.close()
+
+        def check(res, s, what):
+            for q, (r, (w0, Z0)) in enumerate(zip(res, eager[s])):
+                assert np.array_equal(to_host(r.w), w0), f"{what} (set {s}), problem {q}: eigenvalues differ"
+                if Z0 is not None:
+                    assert np.array_equal(to_host(r.Z), Z0), f"{what} (set {s}), problem {q}: vectors differ"
+
+        # creation solved sets[0] eagerly on the graph's buffers
+        torch.cuda.synchronize()
+        check([E.EigResult(w, Z, {}) for w, Z in zip(g.w, g.Z)], 0, "creation")
+        for rep, s in enumerate([1, 2, 0, 1]):
+            check(g.solve(sets[s]), s, f"replay {rep}")
+            if rep == 1:
+                E.syevd(own_ctx, sets[2][4], b=32, nb=256, jobz="V")
+                torch.cuda.synchronize()
+        check(g.replay(), 1, "replay on resident inputs")
+        # accuracy of one problem of the last replay against the FP64 oracle
+        M = to_host(sets[1][7]).astype(np.float64)
+        wref = np.linalg.eigvalsh(M)
+        assert np.abs(to_host(g.w[7]) - wref).max() <= 1e-5 * np.abs(wref).max()
+    finally:
+        g.close()
