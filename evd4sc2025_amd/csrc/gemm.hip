@@ -765,24 +765,37 @@ int syr2k_lower_mirror_batched(evd_ctx *ctx, int count, long n, long k, float al
 // product keeps about FP32 accuracy while running on the 16-bit matrix cores.
 // LDS holds each term k-contiguous per row ([row][16 k], padded to 20), so a
 // lane reads its 4 consecutive k of one row with a single 8-byte load, the
-// operand layout of v_mfma_f32_16x16x16{f16,bf16}.
+// operand layout of v_mfma_f32_16x16x16{f16,bf16}.  Two cross products of one
+// k-slab share a v_mfma_f32_16x16x32_{f16,bf16}: its 8 K slots per lane take
+// the 4 k of one product followed by the 4 k of the other, e.g. A = [lo, hi]
+// against B = [hi, hi] sums lo*hi + hi*hi.  On gfx950 the 16x16x32 forms take
+// the cycles of one 16x16x16, so FP16 issues 1.5 and BF16 3 instead of 3 and
+// 6 instructions per slab.
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int kLDK = 20;
 
 struct SplitF16 {
-  static constexpr int NT = 2, NP = 3;
+  static constexpr int NT = 2;
+  static constexpr int kMinBlocks = 3;  // workgroups per CU the registers are held to
   typedef _Float16 elem;
   typedef f16x4 vec;
   __device__ static inline void split(float x, elem (&t)[NT]) {
     t[0] = (_Float16)x;
     t[1] = (_Float16)(x - (float)t[0]);
   }
-  // (A term, B term) of the products, smallest first
-  __device__ static constexpr int pa(int p) { return p == 0 ? 1 : 0; }
-  __device__ static constexpr int pb(int p) { return p == 1 ? 1 : 0; }
-  __device__ static inline f32x4 mfma(vec b, vec a, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(b, a, c, 0, 0, 0);
+  // the products of one slab, smallest first: A = [hi, lo] against
+  // B = [lo, hi] sums hi*lo + lo*hi, then hi*hi on the 16x16x16 form (its
+  // operands are halves of the same register tuples, so nothing is copied)
+  __device__ static inline f32x4 step(const vec (&b)[NT], const vec (&a)[NT], f32x4 c) {
+    const f16x8 aa = __builtin_shufflevector(a[0], a[1], 0, 1, 2, 3, 4, 5, 6, 7);
+    const f16x8 bb = __builtin_shufflevector(b[1], b[0], 0, 1, 2, 3, 4, 5, 6, 7);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb, aa, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_shufflevector(bb, bb, 4, 5, 6, 7),
+                                                 __builtin_shufflevector(aa, aa, 0, 1, 2, 3), c, 0, 0, 0);
   }
 };
 
@@ -794,7 +807,8 @@ __device__ inline short f32_to_bf16_rne(float x) {
 __device__ inline float bf16_to_f32(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
 
 struct SplitBF16 {
-  static constexpr int NT = 3, NP = 6;
+  static constexpr int NT = 3;
+  static constexpr int kMinBlocks = 2;
   typedef short elem;
   typedef bf16x4 vec;
   __device__ static inline void split(float x, elem (&t)[NT]) {
@@ -803,11 +817,26 @@ struct SplitBF16 {
     t[1] = f32_to_bf16_rne(r1);
     t[2] = f32_to_bf16_rne(r1 - bf16_to_f32(t[1]));
   }
-  // products (A, B) smallest first: mm, lh, hl, mh, hm, hh
-  __device__ static constexpr int pa(int p) { return p == 0 ? 1 : p == 1 ? 2 : p == 3 ? 1 : 0; }
-  __device__ static constexpr int pb(int p) { return p == 0 ? 1 : p == 2 ? 2 : p == 4 ? 1 : 0; }
-  __device__ static inline f32x4 mfma(vec b, vec a, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b, a, c, 0, 0, 0);
+  // (A term, B term) pairs of the three 16x16x32 products, smallest first:
+  // lo*hi + mid*mid, hi*lo + mid*hi, hi*mid + hi*hi
+  __device__ static inline f32x4 mfma2(vec b0, vec b1, vec a0, vec a1, f32x4 c) {
+    const bf16x8s bb = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    const bf16x8s aa = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bb), __builtin_bit_cast(bf16x8, aa), c,
+                                                   0, 0, 0);
+  }
+  __device__ static inline f32x4 step(const vec (&b)[NT], const vec (&a)[NT], f32x4 c) {
+    c = mfma2(b[0], b[1], a[2], a[1], c);
+    c = mfma2(b[2], b[0], a[0], a[1], c);
+    return mfma2(b[1], b[0], a[0], a[0], c);
+  }
+  // the same six products on the 16x16x16 form (fewer registers: the SYR2K
+  // kernel, which holds two accumulator sets, keeps two waves per SIMD)
+  __device__ static inline f32x4 step16(const vec (&b)[NT], const vec (&a)[NT], f32x4 c) {
+    constexpr int pa[6] = {1, 2, 0, 1, 0, 0}, pb[6] = {1, 0, 2, 0, 1, 0};  // mm, lh, hl, mh, hm, hh
+#pragma unroll
+    for (int p = 0; p < 6; p++) c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[pb[p]], a[pa[p]], c, 0, 0, 0);
+    return c;
   }
 };
 
@@ -849,8 +878,9 @@ __device__ inline void store_split(const TileLoader<TA, TB, BM, BN, WM, WN, floa
   }
 }
 
-// split-precision counterpart of tile_mainloop (kBK = 16 = one 16x16x16 MFMA)
-template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN>
+// split-precision counterpart of tile_mainloop (kBK = 16 k per slab; PAIR:
+// two products per 16x16x32 instruction, else S::step16)
+template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN, bool PAIR = true>
 __device__ inline void tile_mainloop_split(const float *__restrict__ A, long lda, const float *__restrict__ B, long ldb,
                                            int M, int N, int m0, int n0, int kb, int ke, typename S::elem *As0,
                                            typename S::elem *Bs0, f32x4 (&acc)[BM / WM / 16][BN / WN / 16]) {
@@ -885,18 +915,27 @@ __device__ inline void tile_mainloop_split(const float *__restrict__ A, long lda
             &Bs[u * BN * kLDK + (wn * (BN / WN) + ni * 16 + (lane & 15)) * kLDK + ko]);
     }
 #pragma unroll
-    for (int p = 0; p < S::NP; p++)
+    for (int mi = 0; mi < MI; mi++)
 #pragma unroll
-      for (int mi = 0; mi < MI; mi++)
+      for (int ni = 0; ni < NI; ni++) {
+        typename S::vec av[S::NT], bv[S::NT];
 #pragma unroll
-        for (int ni = 0; ni < NI; ni++) acc[mi][ni] = S::mfma(b[S::pb(p)][ni], a[S::pa(p)][mi], acc[mi][ni]);
+        for (int u = 0; u < S::NT; u++) {
+          av[u] = a[u][mi];
+          bv[u] = b[u][ni];
+        }
+        if constexpr (PAIR)
+          acc[mi][ni] = S::step(bv, av, acc[mi][ni]);
+        else
+          acc[mi][ni] = S::step16(bv, av, acc[mi][ni]);
+      }
     if (t + 1 < nk) store_split<S>(ld, As0 + (cur ^ 1) * SA, Bs0 + (cur ^ 1) * SB, tid);
     __syncthreads();
   }
 }
 
 template <class S, bool TA, bool TB, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(256) void sgemm_split_kernel(int M, int N, int K, float alpha,
+__global__ __launch_bounds__(256, S::kMinBlocks) void sgemm_split_kernel(int M, int N, int K, float alpha,
                                                           const float *__restrict__ A, long lda, long sA,
                                                           const float *__restrict__ B, long ldb, long sB, float beta,
                                                           float *__restrict__ C, long ldc, long sC, int kchunk,
@@ -961,8 +1000,9 @@ __global__ __launch_bounds__(256) void syr2k_split_kernel(int n, int k, float al
   for (int mi = 0; mi < MI; mi++)
 #pragma unroll
     for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
-  tile_mainloop_split<S, false, true, BM, BN, WM, WN>(A, lda, B, ldb, n, n, m0, n0, 0, k, As, Bs, acc);
-  tile_mainloop_split<S, false, true, BM, BN, WM, WN>(B, ldb, A, lda, n, n, m0, n0, 0, k, As, Bs, acc);
+  constexpr bool pair = S::NT == 2;
+  tile_mainloop_split<S, false, true, BM, BN, WM, WN, pair>(A, lda, B, ldb, n, n, m0, n0, 0, k, As, Bs, acc);
+  tile_mainloop_split<S, false, true, BM, BN, WM, WN, pair>(B, ldb, A, lda, n, n, m0, n0, 0, k, As, Bs, acc);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM;
 #pragma unroll
