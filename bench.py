@@ -369,6 +369,52 @@ def batched_form_bench(E, torch, count: int = 512, seed: int = 1):
             "speedup": t_single / t_mixed}
 
 
+def precision_form_bench(E, torch, n: int = 32768, b: int = 64, nb: int = 2048):
+    """The precision form: n = 32768, FP32 storage, b = 64, nb = 2048, a
+    spectrum of 16 clusters of 2048 eigenvalues (centres uniform [-1, 1],
+    spread 1e-6) on a basis of Gaussian Householder reflectors, solved with
+    split-FP16 and split-BF16 GEMMs (one untimed warm-up solve first).  The
+    exact eigenvalues are the prescribed ones; backward error and
+    orthogonality are taken in FP64 on the GPU."""
+    import numpy as np
+
+    rng = np.random.default_rng(4)
+    centres = rng.uniform(-1.0, 1.0, 16)
+    lam = np.concatenate([c + 1e-6 * rng.uniform(-1.0, 1.0, n // 16) for c in centres])
+    ctx = E.Context(0)
+    out = {"workload": f"n={n} b={b} nb={nb}, 16 clusters of {n // 16} eigenvalues (spread 1e-6), "
+                       "Householder-product basis, FP32 storage"}
+    try:
+        dA = E.gen_spectrum(ctx, torch.from_numpy(lam).cuda(), 21)
+        lam = np.sort(lam)
+        ctx.enable_timing(True)
+        ctx.set_precision(E.PREC_FP16)
+        E.syevd(ctx, dA, b=b, nb=nb)  # warm-up: arena, merge plans
+        for name, prec in (("fp16", E.PREC_FP16), ("bf16", E.PREC_BF16)):
+            ctx.set_precision(prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = E.syevd(ctx, dA, b=b, nb=nb)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            stages = ctx.stage_ms()
+            ctx.set_precision(E.PREC_FP32)
+            w = r.w.cpu().numpy()
+            out[name] = {"s_per_evd": dt, "stage_ms": stages,
+                         "eig_err": float(np.abs(w - lam).max() / np.abs(lam).max()),
+                         "backward_error": float(E.backward_error(ctx, dA, r.w, r.Z)),
+                         "orthogonality": float(E.checkOrthogonality(ctx, r.Z))}
+            del r
+            print(f"[bench] precision form {name}: {dt:.3f} s/EVD", file=sys.stderr, flush=True)
+        del dA
+    finally:
+        ctx.set_precision(E.PREC_FP32)
+        ctx.close()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,6 +431,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--no-captured", action="store_true", help="skip the eager-vs-graph small-solve sample")
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-size batched sample")
+    ap.add_argument("--no-precision-form", action="store_true",
+                    help="skip the n=32768 b=64 split-FP16 / split-BF16 solves")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--cpu-full", action="store_true",
                     help="time host MKL dsyevd on the headline matrix itself (minutes) instead of the n=8192 sample")
@@ -527,6 +575,9 @@ def main():
     if not args.no_batched:
         batched = batched_form_bench(E, torch)
         print(f"[bench] batched form: {batched}", file=sys.stderr, flush=True)
+    precision_form = None
+    if not args.no_precision_form:
+        precision_form = precision_form_bench(E, torch)
     serving = None
     if not args.no_serve:
         serving = serving_bench(E, torch, args.serve_requests)
@@ -559,6 +610,7 @@ def main():
         "serving": serving,
         "captured": captured,
         "batched_form": batched,
+        "precision_form": precision_form,
     }
     print(json.dumps(line))
 

@@ -382,17 +382,23 @@ static int launch_bcback(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, c
 size_t bc_back_wy_bytes(long n);
 bool bc_back_wy_fits(long n, long ldq);
 int bc_back_wy(evd_ctx *ctx, long n, long ncols, float *Q, long ldq, const float *U, long ldu, bool split16);
+size_t bc_back_wy64_bytes(long n);
+bool bc_back_wy64_fits(long n, long ldq);
+int bc_back_wy64_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride,
+                         const float *U, long ldu, long u_stride);
 
 // Q <- H_1 .. H_K Q with b = 32 runs as WY blocks on the matrix cores
 // (bcback_mfma.hip): split FP16 in the FP16 mixed-precision mode, FP32 MFMA
 // otherwise; the other order and b = 64 run on the FP32 VALU kernel below.
 static bool use_wy(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
-  return rev && b == 32 && !ctx->tune.bc_valu && bc_back_wy_fits(n, ldq);
+  if (!rev || ctx->tune.bc_valu) return false;
+  return (b == 32 && bc_back_wy_fits(n, ldq)) || (b == 64 && bc_back_wy64_fits(n, ldq));
 }
 
 size_t bc_back_bytes(const evd_ctx *ctx, long n, long b, long ldq, bool rev) {
   if (b != 32 && b != 64) return 0;  // the small-order kernel needs no workspace
-  return use_wy(ctx, n, b, ldq, rev) ? bc_back_wy_bytes(n) : bc_back_pack_bytes(n, b);
+  if (!use_wy(ctx, n, b, ldq, rev)) return bc_back_pack_bytes(n, b);
+  return b == 64 ? bc_back_wy64_bytes(n) : bc_back_wy_bytes(n);
 }
 
 // Small orders with any bandwidth b <= 64 (n <= kSmallN, the small buckets
@@ -464,7 +470,10 @@ int bc_back_small_batched(evd_ctx *ctx, int count, long n, long b, long ncols, f
 
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev) {
   if (b != 32 && b != 64) return bc_back_small_batched(ctx, 1, n, b, ncols, Q, ldq, 0, U, ldu, 0, rev);
-  if (use_wy(ctx, n, b, ldq, rev)) return bc_back_wy(ctx, n, ncols, Q, ldq, U, ldu, ctx->precision == EVD_PREC_FP16);
+  if (use_wy(ctx, n, b, ldq, rev)) {
+    if (b == 64) return bc_back_wy64_batched(ctx, 1, n, ncols, Q, ldq, 0, U, ldu, 0);
+    return bc_back_wy(ctx, n, ncols, Q, ldq, U, ldu, ctx->precision == EVD_PREC_FP16);
+  }
   const int bs = ctx->tune.bc_sweeps;
   const int nw = ctx->tune.bc_waves;
   if (b == 32) {

@@ -485,6 +485,309 @@ __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *_
   }
 }
 
+// ------------------------------------------------------------------ b = 64
+// The same pipeline for reflectors of length 64 (FP32 MFMA).  Block (S, j)
+// holds the 64 reflectors of sweeps 64S-1 .. 64S+62 at step j, a 128 x 64
+// staircase V (column t nonzero in window rows t .. t+63) on the 128-row
+// window that starts at row 64(S + j); blocks run S descending, j ascending,
+// as above.  A workgroup owns 32 columns, so a half window (64 rows x 32
+// columns) holds as many values as the b = 32 kernel's (32 x 64) and the LDS
+// plan (hand-off slots, staging) is the same.  A block's operands are the
+// nonzero 16 x 16 tiles of V^T (20: sweep tile mt meets window tiles mt ..
+// mt+4) and of -V T (26: column t of V T reaches rows 0 .. t+63), 46 KiB in
+// FP32 fragments, all held in registers while the block's products run.
+constexpr int kB64 = 64;
+constexpr int kBlockFrags64 = 46;
+
+__host__ __device__ inline long wy64_sweep_blocks(long nsweeps) { return (nsweeps + kB64) / kB64; }
+
+// V^T tiles of window row tile rt: sweep tiles vlo(rt) .. vhi(rt), stored
+// from fragment voff(rt); -V T tiles of row tile rt: sweep tiles wlo(rt) .. 3,
+// from fragment woff(rt)
+// (closed forms, not recursion: the device code must fold them to constants)
+__host__ __device__ constexpr int vlo64(int rt) { return rt > 4 ? rt - 4 : 0; }
+__host__ __device__ constexpr int vhi64(int rt) { return rt < 3 ? rt : 3; }
+__host__ __device__ constexpr int voff64(int rt) {
+  return rt <= 4 ? rt * (rt + 1) / 2 : 10 + (rt - 4) * (13 - rt) / 2;  // 0 1 3 6 10 14 17 19
+}
+__host__ __device__ constexpr int wlo64(int rt) { return rt > 4 ? rt - 4 : 0; }
+__host__ __device__ constexpr int woff64(int rt) {
+  return rt <= 5 ? 20 + 4 * rt : 40 + (rt - 5) * (12 - rt) / 2;  // 20 24 .. 40 43 45
+}
+static_assert(voff64(1) == 1 && voff64(4) == 10 && voff64(5) == 14 && voff64(6) == 17 && voff64(7) == 19 &&
+                  woff64(5) == 40 && woff64(6) == 43 && woff64(7) == 45,
+              "fragment table");
+
+// One workgroup per block (S, j): V (FP32, as stored), G = strict upper part
+// of V^T V and T = (I + G)^-1 in FP64, then the fragments; -V T is formed per
+// fragment element.  Fragment q of a block (f4 per lane, component i):
+//   q < 20:  V^T tile (rt, mt): V[16 rt + 4 g + i][16 mt + m]
+//   q >= 20: -V T tile (rt, st): -(V T)[16 rt + m][16 st + 4 g + i]
+// (g = lane / 16, m = lane % 16), the operand layouts of apply_block64.
+__global__ __launch_bounds__(256) void bcw64_prep_kernel(int n, const float *__restrict__ U, long ldu, int nsweeps,
+                                                         f4 *__restrict__ F, long u_stride, long f_stride) {
+  U += blockIdx.z * u_stride;
+  F += blockIdx.z * f_stride;
+  const long Jn = (n + kB64 - 1) / kB64;
+  const int S = blockIdx.y, j = blockIdx.x;
+  if (j >= wy_steps(Jn, S)) return;
+  __shared__ float V[128][65];
+  __shared__ double G[64][65];
+  __shared__ double T[64][65];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < 128 * 64; idx += 256) {
+    const int r = idx & 127, t = idx >> 7, i = r - t;
+    const long sw = (long)S * kB64 - 1 + t;
+    const long row = (long)S * kB64 + (long)j * kB64 + r;
+    float v = 0.f;
+    if (i >= 0 && i < kB64 && sw >= 0 && sw < nsweeps && row < n) v = U[row + sw * ldu];
+    V[r][t] = v;
+  }
+  __syncthreads();
+  // G[a][c] = v_a . v_c for a < c: the columns overlap in rows c .. a+63
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int a = idx & 63, c = idx >> 6;
+    double s = 0.0;
+    if (a < c)
+      for (int r = c; r < a + kB64; r++) s += (double)V[r][a] * (double)V[r][c];
+    G[a][c] = s;
+  }
+  __syncthreads();
+  // T = (I + G)^-1, column c by back substitution (one thread per column)
+  if (tid < kB64) {
+    const int c = tid;
+    for (int i = 0; i < kB64; i++) T[i][c] = (i == c) ? 1.0 : 0.0;
+    for (int i = c - 1; i >= 0; i--) {
+      double s = 0.0;
+      for (int q = i + 1; q <= c; q++) s += G[i][q] * T[q][c];
+      T[i][c] = -s;
+    }
+  }
+  __syncthreads();
+  const long blk = wy_prefix(Jn, S) + j;
+  f4 *out = F + blk * (long)(kBlockFrags64 * kFrag);
+  for (int idx = tid; idx < kBlockFrags64 * kFrag; idx += 256) {
+    const int q = idx >> 6, lane = idx & 63, g = lane >> 4, m = lane & 15;
+    f4 v;
+    if (q < 20) {
+      int rt = 0;
+      while (rt < 7 && voff64(rt + 1) <= q) rt++;
+      const int mt = vlo64(rt) + q - voff64(rt);
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[i] = V[16 * rt + 4 * g + i][16 * mt + m];
+    } else {
+      int rt = 0;
+      while (rt < 7 && woff64(rt + 1) <= q) rt++;
+      const int st = wlo64(rt) + q - woff64(rt);
+      const int r = 16 * rt + m;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int t = 16 * st + 4 * g + i;
+        double acc = 0.0;
+        for (int p = 0; p <= t; p++) acc += (double)V[r][p] * T[p][t];
+        v[i] = (float)-acc;
+      }
+    }
+    out[idx] = v;
+  }
+}
+
+// Element (row r, column c) of a 64 x 32 half window in accumulator order
+// (tile 2 rt + nt, lane 16 g + c16, component i), as a float index.
+__device__ __attribute__((always_inline)) inline int half64_index(int r, int c) {
+  return (((r >> 4) * 2 + (c >> 4)) * 64 + ((r >> 2) & 3) * 16 + (c & 15)) * 4 + (r & 3);
+}
+
+__device__ __attribute__((always_inline)) inline void put_half64(f4 *slot, const f4 (&h)[4][2], int lane) {
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) slot[(rt * 2 + nt) * 64 + lane] = h[rt][nt];
+}
+__device__ __attribute__((always_inline)) inline void get_half64(f4 (&h)[4][2], const f4 *slot, int lane) {
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) h[rt][nt] = slot[(rt * 2 + nt) * 64 + lane];
+}
+
+// HBM <-> LDS traffic of a 64 x 32 half window over the 4 waves: wave k moves
+// columns 8k .. 8k+7, lane = row, one instruction = one column of 64 rows.
+// Skipped rows and columns get an offset past the buffer (dropped stores,
+// zero loads), so every step issues the same memory operations.
+struct HalfIO64 {
+  __amdgpu_buffer_rsrc_t rs;
+  int r, c;         // row of the lane, first column of the wave (within the 32)
+  unsigned cbytes;  // byte offset of the wave's first column
+  int ncl;          // columns of this workgroup
+  __device__ HalfIO64(float *Zc, long ldz, int n, int ncols_wg, int lane, int k)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(Zc, 0, (int)(((long)(ncols_wg - 1) * ldz + n) * 4), kBufWord3)),
+        r(lane),
+        c(8 * k),
+        cbytes((unsigned)(c * ldz * 4)),
+        ncl(ncols_wg) {}
+  __device__ __attribute__((always_inline)) inline unsigned off(bool on, long row0, int n, long ldz, int q) const {
+    const long gr = row0 + r;
+    return (on && gr < n && c + q < ncl) ? cbytes + (unsigned)((q * ldz + gr) * 4) : kOutside;
+  }
+  __device__ __attribute__((always_inline)) inline void load(float (&v)[8], bool on, long row0, int n, long ldz) const {
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off(on, row0, n, ldz, q), 0, 0));
+  }
+  __device__ __attribute__((always_inline)) inline void park(f4 *slot, const float (&v)[8]) const {
+    float *s = reinterpret_cast<float *>(slot);
+#pragma unroll
+    for (int q = 0; q < 8; q++) s[half64_index(r, c + q)] = v[q];
+  }
+  __device__ __attribute__((always_inline)) inline void store(const f4 *slot, bool on, long row0, int n,
+                                                              long ldz) const {
+    const float *s = reinterpret_cast<const float *>(slot);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, s[half64_index(r, c + q)]), rs,
+                                            off(on, row0, n, ldz, q), 0, 0);
+  }
+};
+
+// fragments q0 .. q0 + N - 1 of a block into fr[0 .. N-1]
+template <int N, int M>
+__device__ __attribute__((always_inline)) inline void load_tiles(f4 (&fr)[M], const f4 *__restrict__ blk, int q0,
+                                                                 int lane) {
+#pragma unroll
+  for (int q = 0; q < N; q++) fr[q] = blk[(q0 + q) * kFrag + lane];
+}
+
+// Zw <- (I - V T V^T) Zw for one block.  vt holds the block's 20 V^T tiles
+// (loaded one block ahead); the 26 -V T tiles are loaded here, behind the X
+// products; once X is complete, the next block's V^T tiles (nxt, may be null)
+// are loaded into vt behind the Zw products.  All operands are resident while
+// the products run, so the loops are ordered for independent accumulators:
+// X over row tiles 3, 4, 2, 5, 1, 6, 0, 7 (consecutive tiles feed disjoint or
+// wide sets of X tiles), Zw over all 16 window tiles per K step.
+__device__ __attribute__((always_inline)) inline void apply_block64(f4 (&top)[4][2], f4 (&bot)[4][2], f4 (&vt)[20],
+                                                                    const f4 *__restrict__ blk,
+                                                                    const f4 *__restrict__ nxt, int lane) {
+  f4 wt[26];
+  load_tiles<26>(wt, blk, 20, lane);
+  __builtin_amdgcn_sched_barrier(0);
+  f4 x[4][2];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) x[mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
+  // (every loop has a constant trip count, so all indices fold after unrolling
+  // and the fragment arrays stay in registers)
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int oi = 0; oi < 8; oi++) {
+      const int rt = (oi & 1) ? 4 + (oi >> 1) : 3 - (oi >> 1);  // 3, 4, 2, 5, 1, 6, 0, 7
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+          if (mt < vlo64(rt) || mt > vhi64(rt)) continue;
+          x[mt][nt] = mma(vt[voff64(rt) + mt - vlo64(rt)][i], (rt < 4 ? top[rt & 3][nt] : bot[rt & 3][nt])[i],
+                          x[mt][nt]);
+        }
+    }
+  __builtin_amdgcn_sched_barrier(0);
+  if (nxt) load_tiles<20>(vt, nxt, 0, lane);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int st = 0; st < 4; st++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int rt = 0; rt < 8; rt++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+          if (st < wlo64(rt)) continue;
+          const f4 &w = wt[woff64(rt) - 20 + st - wlo64(rt)];
+          if (rt < 4)
+            top[rt & 3][nt] = mma(w[i], x[st][nt][i], top[rt & 3][nt]);
+          else
+            bot[rt & 3][nt] = mma(w[i], x[st][nt][i], bot[rt & 3][nt]);
+        }
+}
+
+// The b = 64 pipeline: as bcw_kernel (waves over consecutive sweep blocks,
+// rows handed on through LDS), with 32 columns per workgroup and 4 waves, one
+// per SIMD: a wave's window, X and a block's fragments need more than the 256
+// registers two waves per SIMD leave each.  Q is streamed from HBM once per
+// 256 sweeps.  Per step the fragment loads come first and the HBM row traffic
+// after the products, so waiting for a fragment never waits for HBM.
+__global__ __launch_bounds__(256) void bcw64_kernel(int n, int ncols, float *__restrict__ Z, long ldz,
+                                                    const f4 *__restrict__ F, int nsweeps, long z_stride,
+                                                    long f_stride) {
+  constexpr int NW = 4;
+  Z += blockIdx.y * z_stride;
+  F += blockIdx.y * f_stride;
+  __shared__ f4 xch[NW * 2 * 512];
+  __shared__ f4 stage[2 * 512];
+  __shared__ f4 stage_top[2 * 512];
+  const int lane = threadIdx.x & 63;
+  const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col0 = blockIdx.x * 32;
+  const int nblk = (int)wy64_sweep_blocks(nsweeps);
+  const long Jn = (n + kB64 - 1) / kB64;
+  const HalfIO64 io(Z + (long)col0 * ldz, ldz, n, min(32, ncols - col0), lane, k);
+  f4 zt[4][2], zb[4][2];
+  f4 vt[20];
+
+  const int ngroups = (nblk + NW - 1) / NW;
+  for (int gi = 0; gi < ngroups; gi++) {
+    const int gtop = nblk - 1 - gi * NW;
+    const int nact = min(NW, gtop + 1);
+    const int bk = gtop - k;
+    const int J = (int)wy_steps(Jn, gtop - nact + 1) + 1;
+    const int T = J + nact - 1;
+    const int kl = nact - 1;
+    const long nst = wy_steps(Jn, bk);
+    const f4 *fb = F + wy_prefix(Jn, bk) * (long)(kBlockFrags64 * kFrag);
+    auto body = [&](int tau, float(&lb)[8], float(&lt)[8], const float(&pb)[8], const float(&pt)[8])
+                    __attribute__((always_inline)) {
+      const int e = tau - k;
+      const bool act = tau >= 0 && tau < T && k < nact && e >= 0 && e < J;
+      const bool app = act && e < nst;
+      const f4 *cur = fb + (long)e * (kBlockFrags64 * kFrag);
+      if (app && e == 0) load_tiles<20>(vt, cur, 0, lane);
+      if (act) {
+        if (e == 0) {
+          get_half64(zt, stage_top + (k & 1) * 512, lane);
+        } else {
+#pragma unroll
+          for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) zt[rt][nt] = zb[rt][nt];
+        }
+        get_half64(zb, k == 0 ? stage + (e & 1) * 512 : xch + ((k - 1) * 2 + (e & 1)) * 512, lane);
+        if (app) apply_block64(zt, zb, vt, cur, (e + 1 < nst && e + 1 < J) ? cur + kBlockFrags64 * kFrag : nullptr, lane);
+        put_half64(xch + (k * 2 + (e & 1)) * 512, zt, lane);
+      }
+      // rows the last wave handed on at step tau-1 -> Q
+      const int el = tau - 1 - kl;
+      io.store(xch + (kl * 2 + (el & 1)) * 512, el >= 0 && el < J, (long)(gtop - kl + el) * kB64, n, ldz);
+      // rows needed at step tau+2: wave 0's new bottom half, and the top half
+      // of wave tau+2's first step
+      io.load(lb, tau + 2 < J, (long)(gtop + tau + 3) * kB64, n, ldz);
+      io.load(lt, tau + 2 < nact, (long)(gtop - tau - 2) * kB64, n, ldz);
+      // park the rows loaded at step tau-1 (for step tau+1)
+      if (tau >= -1 && tau + 1 < J) io.park(stage + ((tau + 1) & 1) * 512, pb);
+      if (tau >= -1 && tau + 1 < nact) io.park(stage_top + ((tau + 1) & 1) * 512, pt);
+      lds_barrier();
+    };
+    float la[8], ta[8], lb2[8], tb2[8];
+    for (int tau = -2; tau <= T; tau += 2) {
+      body(tau, la, ta, lb2, tb2);
+      if (tau + 1 <= T) body(tau + 1, lb2, tb2, la, ta);
+    }
+    wait_mem();
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 size_t bc_back_wy_bytes(long n) {
@@ -526,6 +829,38 @@ int bc_back_wy_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, lo
   else
     bcw_kernel<NW, true><<<grid, NW * 64, 0, ctx->stream>>>((int)n, (int)ncols, Q, ldq, F, nsweeps, q_stride,
                                                              f_stride);
+  EVD_LAUNCHED(ctx);
+  if (ctx->timing) hipEventRecord(ctx->kev[1], ctx->stream);
+  return 0;
+}
+
+size_t bc_back_wy64_bytes(long n) {
+  if (n <= 2) return 0;
+  return align256((size_t)wy_prefix((n + kB64 - 1) / kB64, wy64_sweep_blocks(n - 2)) * kBlockFrags64 * kFrag *
+                  sizeof(f4));
+}
+
+// the 32 columns of a workgroup must be addressable with 32-bit buffer offsets
+bool bc_back_wy64_fits(long n, long ldq) { return (31 * ldq + n) * 4 < (1L << 31); }
+
+// Q <- H_1 .. H_K Q for b = 64 as WY blocks on FP32 MFMA (every precision
+// form), a batch of `count` same-size problems per launch.
+int bc_back_wy64_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride,
+                         const float *U, long ldu, long u_stride) {
+  const int nsweeps = (int)std::max<long>(0, n - 2);
+  if (nsweeps == 0 || ncols <= 0 || count <= 0) return 0;
+  ArenaFrame frame(ctx);
+  const long fbytes = (long)bc_back_wy64_bytes(n);
+  f4 *F = static_cast<f4 *>(arena_alloc(ctx, (size_t)fbytes * count));
+  if (!F) return EVD_E_NOMEM;
+  const long f_stride = fbytes / (long)sizeof(f4);
+  const long Jn = (n + kB64 - 1) / kB64, nblk = wy64_sweep_blocks(nsweeps);
+  bcw64_prep_kernel<<<dim3((unsigned)Jn, (unsigned)nblk, (unsigned)count), 256, 0, ctx->stream>>>(
+      (int)n, U, ldu, nsweeps, F, u_stride, f_stride);
+  EVD_LAUNCHED(ctx);
+  if (ctx->timing) hipEventRecord(ctx->kev[0], ctx->stream);
+  bcw64_kernel<<<dim3((unsigned)((ncols + 31) / 32), (unsigned)count), 256, 0, ctx->stream>>>(
+      (int)n, (int)ncols, Q, ldq, F, nsweeps, q_stride, f_stride);
   EVD_LAUNCHED(ctx);
   if (ctx->timing) hipEventRecord(ctx->kev[1], ctx->stream);
   return 0;

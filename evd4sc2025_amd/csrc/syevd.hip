@@ -38,6 +38,10 @@ size_t sb2st_batched_workspace(int count, long n);
 bool bc_back_wy_fits(long n, long ldq);
 int bc_back_wy_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride, const float *U,
                        long ldu, long u_stride, bool split16);
+size_t bc_back_wy64_bytes(long n);
+bool bc_back_wy64_fits(long n, long ldq);
+int bc_back_wy64_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, long ldq, long q_stride,
+                         const float *U, long ldu, long u_stride);
 int bc_back(evd_ctx *ctx, long n, long b, long ncols, float *Q, long ldq, const float *U, long ldu, bool rev);
 int bc_back_small_batched(evd_ctx *ctx, int count, long n, long b, long ncols, float *Q, long ldq, long q_stride,
                           const float *U, long ldu, long u_stride, bool rev);
@@ -194,6 +198,7 @@ size_t syevd_workspace(long n, long b, long nb, char jobz, int precision, long s
                   ((size_t)64 << 20);
     back = std::max(back, bc_back_pack_bytes(n, b) + ((size_t)1 << 20));
     if (b == 32) back = std::max(back, bc_back_wy_bytes(n) + ((size_t)1 << 20));
+    if (b == 64) back = std::max(back, bc_back_wy64_bytes(n) + ((size_t)1 << 20));
     later = std::max(later, std::max(dc, back));
   }
   return s + later + ((size_t)4 << 20);
@@ -286,6 +291,7 @@ size_t syevd_batched_workspace(int count, long n, long b, long nb, char jobz, in
                               (align256((size_t)(wc / 2) * blk * sizeof(float)) +
                                align256((size_t)blk * n * sizeof(float))) * (size_t)count);
     if (b == 32) back = std::max(back, bc_back_wy_bytes(n) * count + ((size_t)1 << 20));
+    if (b == 64) back = std::max(back, bc_back_wy64_bytes(n) * count + ((size_t)1 << 20));
     later = std::max(later, back);
   }
   return s + later + ((size_t)4 << 20);
@@ -339,6 +345,8 @@ int syevd_batched(evd_ctx *ctx, char jobz, int count, long n, long b, long nb, f
     } else if (b == 32 && !ctx->tune.bc_valu && bc_back_wy_fits(n, ldz)) {
       EVD_TRY(bc_back_wy_batched(ctx, count, n, n, Z, ldz, z_stride, U, ldu, u_stride,
                                  ctx->precision == EVD_PREC_FP16));
+    } else if (b == 64 && !ctx->tune.bc_valu && bc_back_wy64_fits(n, ldz)) {
+      EVD_TRY(bc_back_wy64_batched(ctx, count, n, n, Z, ldz, z_stride, U, ldu, u_stride));
     } else {
       for (int q = 0; q < count; q++)
         EVD_TRY(bc_back(ctx, n, b, n, Z + q * z_stride, ldz, U + q * u_stride, ldu, true));

@@ -173,10 +173,10 @@ extern "C" int evd_autotune(evd_ctx *ctx, long n, long b, int reps, int *out_kno
         {&ctx->tune.sbr_back_block, {0, 512, 2048}, EVD_STAGE_SBRBACK, EVD_STAGE_SBRBACK + 1},
     };
     for (Knob &kb : knobs) {
-      if (kb.slot == &ctx->tune.bc_valu && b != 32) continue;  // one BC-back kernel for b = 64
       if (kb.slot == &ctx->tune.chase_depth && b > 32) continue;  // the grouped chase needs b <= 32
-      // the VALU BC back transformation's shape, when that kernel runs
-      const bool valu_bc = (b == 64 || ctx->tune.bc_valu == 1);
+      // the VALU BC back transformation's shape, when that kernel runs (its
+      // b = 64 form has one shape)
+      const bool valu_bc = ctx->tune.bc_valu == 1 && b == 32;
       if ((kb.slot == &ctx->tune.bc_sweeps || kb.slot == &ctx->tune.bc_waves) && !valu_bc) continue;
       int best_v = *kb.slot;
       float best_t = 1e30f;

@@ -313,10 +313,13 @@ def test_stedc_library_products_match_grouped_kernel(ctx, n):
 
 @pytest.mark.parametrize("valu", [0, 1])
 @pytest.mark.parametrize("n,b,order", [(70, 32, 1), (70, 32, 0), (300, 32, 1), (300, 32, 0), (257, 64, 1),
-                                       (257, 64, 0), (1100, 32, 1)])
+                                       (257, 64, 0), (1100, 32, 1), (66, 64, 1), (130, 64, 1), (700, 64, 1),
+                                       (1100, 64, 1)])
 def test_bc_back_matches_oracle(ctx, n, b, order, valu):
-    """order 1 with b = 32 runs as WY blocks on FP32 MFMA (valu = 0) or on the
-    FP32 VALU kernel (valu = 1); the other cases always take the VALU kernel."""
+    """order 1 runs as WY blocks on FP32 MFMA (valu = 0; b = 32 and b = 64
+    kernels) or on the FP32 VALU kernel (valu = 1); order 0 always takes the
+    VALU kernel.  150 columns: the b = 64 kernel's last 32-column workgroup
+    is partial."""
     import evd4sc2025_amd as E
 
     ctx.set_knob("bc_valu", valu)

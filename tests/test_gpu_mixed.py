@@ -143,3 +143,31 @@ def test_bc_back_wy_fp16_large_matches_fp32(mctx, n, ncols):
     # orthogonal transformations keep the column norms at 1
     for key in out:
         assert np.abs(np.linalg.norm(out[key], axis=0) - 1).max() < 5e-6, key
+
+
+@pytest.mark.parametrize("n,ncols", [(4100, 300), (2113, 33)])
+def test_bc_back_wy64_large_matches_valu(mctx, n, ncols):
+    """b = 64 reflectors from the GPU band reduction and chase, beyond the
+    oracle's reach: the WY kernel (128 x 64 staircase blocks on FP32 MFMA)
+    against the FP32 VALU kernel, and column norms kept at 1."""
+    import evd4sc2025_amd as E
+
+    torch = pytest.importorskip("torch")
+    mctx.set_precision(E.PREC_FP32)
+    A0 = E.gen_symmetric(mctx, n, 6, 0)
+    Ab, _, _ = E.sy2sb(mctx, A0, 64, 256)
+    band = E.band_pack(mctx, Ab, 64)
+    _, _, U = E.sb2st(mctx, band, n, 64)
+    X = _unit_columns(n, ncols, 8)
+    out = {}
+    for key, valu in (("valu", 1), ("wy64", 0)):
+        mctx.set_knob("bc_valu", valu)
+        dX = to_dev(X)
+        E.bc_back(mctx, dX, U, n, 64, 1)
+        torch.cuda.synchronize()
+        out[key] = to_host(dX).astype(np.float64)
+    mctx.set_knob("bc_valu", 0)
+    d = np.linalg.norm(out["wy64"] - out["valu"], axis=0).max()
+    assert d < 5e-6, d
+    for key in out:
+        assert np.abs(np.linalg.norm(out[key], axis=0) - 1).max() < 5e-6, key
