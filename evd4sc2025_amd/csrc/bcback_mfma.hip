@@ -235,10 +235,11 @@ __device__ __attribute__((always_inline)) inline void load_frags(FR (&fr)[N], co
 
 // Zw <- (I - V T V^T) Zw for one block; top = window rows 0..31, bot = 32..63.
 // v0 / v1 hold V^T for window rows 0..31 / 32..63, wt holds -V T.  With nxt
-// given, the next block's v0 and wt are loaded as soon as the current ones are
-// consumed, so their latency hides behind the rest of the step.
+// given, the next block's v0, v1 and wt are loaded as soon as the current ones
+// are consumed, so their latency hides behind the rest of the step and the
+// next one's start.
 __device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4], f4 (&bot)[2][4], h8 (&v0)[4],
-                                                                  const h8 (&v1)[4], h8 (&wt)[8],
+                                                                  h8 (&v1)[4], h8 (&wt)[8],
                                                                   const h8 *__restrict__ nxt, int lane) {
   f4 x[2][4];
 #pragma unroll
@@ -271,6 +272,8 @@ __device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4
 #pragma unroll
     for (int mt = 0; mt < 2; mt++) x[mt][nt] = mma(v1[2 * mt], zh, x[mt][nt]);
   }
+  __builtin_amdgcn_sched_barrier(0);
+  if (nxt) load_frags(v1, nxt + 4 * kFrag, lane);
   // Zw += (-V T) X, K = 32 sweeps
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) {
@@ -296,7 +299,7 @@ __device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4
 // The FP32 form of apply_block: V^T and -V T as f4 fragments, K = 4 per
 // v_mfma_f32_16x16x4f32; accumulator tiles are the B operands as they are.
 __device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4], f4 (&bot)[2][4], f4 (&v0)[4],
-                                                                  const f4 (&v1)[4], f4 (&wt)[8],
+                                                                  f4 (&v1)[4], f4 (&wt)[8],
                                                                   const f4 *__restrict__ nxt, int lane) {
   f4 x[2][4];
 #pragma unroll
@@ -326,6 +329,8 @@ __device__ __attribute__((always_inline)) inline void apply_block(f4 (&top)[2][4
 #pragma unroll
         for (int nt = 0; nt < 4; nt++)
           if (!(rt == 1 && mt == 0)) x[mt][nt] = mma(v1[2 * rt + mt][i], bot[rt][nt][i], x[mt][nt]);
+  __builtin_amdgcn_sched_barrier(0);
+  if (nxt) load_frags(v1, nxt + 4 * kFrag, lane);
   // Zw += (-V T) X
 #pragma unroll
   for (int st = 0; st < 2; st++)
@@ -439,14 +444,13 @@ __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *_
       const bool act = tau >= 0 && tau < T && k < nact && e >= 0 && e < J;
       const bool app = act && e < nst;
       const Frag *cur = fb + (long)e * (kBlockFrags * kFrag);
-      // (0) this step's fragments first: waiting for them then never waits
-      // for the HBM traffic issued below
-      if (app) {
-        if (e == 0) {
-          load_frags(v0, cur, lane);
-          load_frags(wt, cur + 8 * kFrag, lane);
-        }
+      // (0) a wave's first block: its fragments before the HBM traffic
+      // issued below, so waiting for them never waits for HBM (later blocks'
+      // fragments are loaded one block ahead inside apply_block)
+      if (app && e == 0) {
+        load_frags(v0, cur, lane);
         load_frags(v1, cur + 4 * kFrag, lane);
+        load_frags(wt, cur + 8 * kFrag, lane);
       }
       // (1) rows the last wave handed on at step tau-1 -> Q
       const int el = tau - 1 - kl;
