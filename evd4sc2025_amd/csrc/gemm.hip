@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float a
 #pragma unroll
     for (int ni = 0; ni < NI; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
   // the long-K skinny products take the two-slab pipeline
-  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK, BM == 256 && BN == 32, KCAT>(
+  tile_mainloop<TA, TB, BM, BN, WM, WN, float, f32x4, BK, BM == 256 && (BN == 32 || BN == 64), KCAT>(
       A, lda, B, ldb, M, N, m0, n0, kb, ke, As, Bs, acc, kp);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -563,6 +563,13 @@ static int sgemm_dispatch(evd_ctx *ctx, long m, long n, long k, float alpha, con
                                                  128, 2, kc);
     return launch_sgemm<TA, TB, 64, 32, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 128,
                                               2, kc);
+  }
+  if (n <= 64 && m > 64 && k >= 4096) {
+    // b = 64 skinny long-K products (Z = A W): 256 x 64 tiles on the
+    // two-slab pipeline, split to 4 workgroups per CU like the b = 32 form
+    // (64 x 64 tiles moved 2.4 TB/s at n = 32768)
+    return launch_sgemm<TA, TB, 256, 64, 4, 1>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                                               256, ctx->tune.skinny_split > 0 ? ctx->tune.skinny_split : 4, kc);
   }
   if (m <= 32 && n > 64)
     return launch_sgemm<TA, TB, 32, 256, 1, 4>(ctx, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,

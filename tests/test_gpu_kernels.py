@@ -22,7 +22,9 @@ def _torch():
      # long-K skinny products (256 x 32 tiles, K split): interior tiles on the
      # two-slab pipeline, an edge tile, a partial last slab, an unaligned lda
      (5000, 32, 4100), (2048, 32, 4096), (4999, 32, 4103),
-     # K splits of up to 8 summed inside the launch (64 x 64 and 64 x 32 tiles)
+     # the same for b = 64 (256 x 64 tiles), and a 48-column edge tile
+     (5000, 64, 4100), (4999, 64, 4103), (300, 48, 5000),
+     # small K splits (64 x 64 and 64 x 32 tiles)
      (96, 96, 1024), (700, 32, 900)])
 def test_sgemm_matches_fp64(ctx, ta, tb, m, n, k):
     import evd4sc2025_amd as E
