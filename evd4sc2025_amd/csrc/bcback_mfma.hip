@@ -494,9 +494,7 @@ __global__ __launch_bounds__(NW * 64) void bcw_kernel(int n, int ncols, float *_
 // holds the 64 reflectors of sweeps 64S-1 .. 64S+62 at step j, a 128 x 64
 // staircase V (column t nonzero in window rows t .. t+63) on the 128-row
 // window that starts at row 64(S + j); blocks run S descending, j ascending,
-// as above.  A workgroup owns 32 columns, so a half window (64 rows x 32
-// columns) holds as many values as the b = 32 kernel's (32 x 64) and the LDS
-// plan (hand-off slots, staging) is the same.  A block's operands are the
+// as above.  A block's operands are the
 // nonzero 16 x 16 tiles of V^T (20: sweep tile mt meets window tiles mt ..
 // mt+4) and of -V T (26: column t of V T reaches rows 0 .. t+63), 46 KiB in
 // FP32 fragments, all held in registers while the block's products run.
@@ -596,61 +594,88 @@ __global__ __launch_bounds__(256) void bcw64_prep_kernel(int n, const float *__r
   }
 }
 
-// Element (row r, column c) of a 64 x 32 half window in accumulator order
-// (tile 2 rt + nt, lane 16 g + c16, component i), as a float index.
+// A workgroup of the b = 64 kernel owns kC64 columns: a half window is 64
+// rows x kC64 columns, kNT64 column tiles of 16.
+constexpr int kC64 = 64, kNT64 = kC64 / 16, kHalf64 = 4 * kNT64 * 64;  // f4 per half window
+
+// Element (row r, column c) of a half window in accumulator order (tile
+// kNT64 rt + nt, lane 16 g + c16, component i), as a float index.
 __device__ __attribute__((always_inline)) inline int half64_index(int r, int c) {
-  return (((r >> 4) * 2 + (c >> 4)) * 64 + ((r >> 2) & 3) * 16 + (c & 15)) * 4 + (r & 3);
+  return (((r >> 4) * kNT64 + (c >> 4)) * 64 + ((r >> 2) & 3) * 16 + (c & 15)) * 4 + (r & 3);
 }
 
-__device__ __attribute__((always_inline)) inline void put_half64(f4 *slot, const f4 (&h)[4][2], int lane) {
+__device__ __attribute__((always_inline)) inline void put_half64(f4 *slot, const f4 (&h)[4][kNT64], int lane) {
 #pragma unroll
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-    for (int nt = 0; nt < 2; nt++) slot[(rt * 2 + nt) * 64 + lane] = h[rt][nt];
+    for (int nt = 0; nt < kNT64; nt++) slot[(rt * kNT64 + nt) * 64 + lane] = h[rt][nt];
 }
-__device__ __attribute__((always_inline)) inline void get_half64(f4 (&h)[4][2], const f4 *slot, int lane) {
+__device__ __attribute__((always_inline)) inline void get_half64(f4 (&h)[4][kNT64], const f4 *slot, int lane) {
 #pragma unroll
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-    for (int nt = 0; nt < 2; nt++) h[rt][nt] = slot[(rt * 2 + nt) * 64 + lane];
+    for (int nt = 0; nt < kNT64; nt++) h[rt][nt] = slot[(rt * kNT64 + nt) * 64 + lane];
 }
 
-// HBM <-> LDS traffic of a 64 x 32 half window over the 4 waves: wave k moves
-// columns 8k .. 8k+7, lane = row, one instruction = one column of 64 rows.
-// Skipped rows and columns get an offset past the buffer (dropped stores,
-// zero loads), so every step issues the same memory operations.
+// HBM traffic of the b = 64 kernel.  Half windows staged through LDS (wave
+// 0's incoming rows, the last wave's finished rows) are spread over the 4
+// waves: wave k moves columns kQ k .. kQ k + kQ - 1, lane = row, one
+// instruction = one column of 64 rows.  Skipped rows and columns get an
+// offset past the buffer (dropped stores, zero loads), so every step issues
+// the same memory operations.  load_acc brings a wave's own first top half
+// straight into its accumulator tiles.
 struct HalfIO64 {
+  static constexpr int kQ = kC64 / 4;
   __amdgpu_buffer_rsrc_t rs;
-  int r, c;         // row of the lane, first column of the wave (within the 32)
+  int r, c;         // row of the lane, first column of the wave (within the workgroup's)
   unsigned cbytes;  // byte offset of the wave's first column
   int ncl;          // columns of this workgroup
   __device__ HalfIO64(float *Zc, long ldz, int n, int ncols_wg, int lane, int k)
       : rs(__builtin_amdgcn_make_buffer_rsrc(Zc, 0, (int)(((long)(ncols_wg - 1) * ldz + n) * 4), kBufWord3)),
         r(lane),
-        c(8 * k),
+        c(kQ * k),
         cbytes((unsigned)(c * ldz * 4)),
         ncl(ncols_wg) {}
   __device__ __attribute__((always_inline)) inline unsigned off(bool on, long row0, int n, long ldz, int q) const {
     const long gr = row0 + r;
     return (on && gr < n && c + q < ncl) ? cbytes + (unsigned)((q * ldz + gr) * 4) : kOutside;
   }
-  __device__ __attribute__((always_inline)) inline void load(float (&v)[8], bool on, long row0, int n, long ldz) const {
+  __device__ __attribute__((always_inline)) inline void load(float (&v)[kQ], bool on, long row0, int n,
+                                                             long ldz) const {
 #pragma unroll
-    for (int q = 0; q < 8; q++)
+    for (int q = 0; q < kQ; q++)
       v[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off(on, row0, n, ldz, q), 0, 0));
   }
-  __device__ __attribute__((always_inline)) inline void park(f4 *slot, const float (&v)[8]) const {
+  __device__ __attribute__((always_inline)) inline void park(f4 *slot, const float (&v)[kQ]) const {
     float *s = reinterpret_cast<float *>(slot);
 #pragma unroll
-    for (int q = 0; q < 8; q++) s[half64_index(r, c + q)] = v[q];
+    for (int q = 0; q < kQ; q++) s[half64_index(r, c + q)] = v[q];
   }
   __device__ __attribute__((always_inline)) inline void store(const f4 *slot, bool on, long row0, int n,
                                                               long ldz) const {
     const float *s = reinterpret_cast<const float *>(slot);
 #pragma unroll
-    for (int q = 0; q < 8; q++)
+    for (int q = 0; q < kQ; q++)
       __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, s[half64_index(r, c + q)]), rs,
                                             off(on, row0, n, ldz, q), 0, 0);
+  }
+  // rows row0 .. row0+63 of all the workgroup's columns in accumulator order:
+  // tile (rt, nt), lane 16 g + c16, component i = row 16 rt + 4 g + i of
+  // column 16 nt + c16
+  __device__ __attribute__((always_inline)) inline void load_acc(f4 (&h)[4][kNT64], long row0, int n, long ldz,
+                                                                 int lane) const {
+    const int c16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+      for (int nt = 0; nt < kNT64; nt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const long gr = row0 + 16 * rt + 4 * g + i;
+          const int col = 16 * nt + c16;
+          const unsigned o = (gr < n && col < ncl) ? (unsigned)((col * ldz + gr) * 4) : kOutside;
+          h[rt][nt][i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o, 0, 0));
+        }
   }
 };
 
@@ -663,81 +688,95 @@ __device__ __attribute__((always_inline)) inline void load_tiles(f4 (&fr)[M], co
 }
 
 // Zw <- (I - V T V^T) Zw for one block.  vt holds the block's 20 V^T tiles
-// (loaded one block ahead); the 26 -V T tiles are loaded here, behind the X
-// products; once X is complete, the next block's V^T tiles (nxt, may be null)
-// are loaded into vt behind the Zw products.  All operands are resident while
-// the products run, so the loops are ordered for independent accumulators:
-// X over row tiles 3, 4, 2, 5, 1, 6, 0, 7 (consecutive tiles feed disjoint or
-// wide sets of X tiles), Zw over all 16 window tiles per K step.
-__device__ __attribute__((always_inline)) inline void apply_block64(f4 (&top)[4][2], f4 (&bot)[4][2], f4 (&vt)[20],
-                                                                    const f4 *__restrict__ blk,
+// (loaded one block ahead).  The loads are placed so that few operand tiles
+// are live at once: the -V T tiles of window row tiles 0..3 (16) load at
+// the start, behind the X products, those of row tiles 4..7 (10) once X has
+// consumed most of vt, and the next block's V^T tiles (nxt, may be null) once
+// the first Zw half has consumed the first -V T part.  Every product loop runs
+// the kNT64 = 4 column tiles as independent accumulators.
+__device__ __attribute__((always_inline)) inline void apply_block64(f4 (&top)[4][kNT64], f4 (&bot)[4][kNT64],
+                                                                    f4 (&vt)[20], const f4 *__restrict__ blk,
                                                                     const f4 *__restrict__ nxt, int lane) {
-  f4 wt[26];
-  load_tiles<26>(wt, blk, 20, lane);
+  f4 wa[16], wb[10];
+  load_tiles<16>(wa, blk, woff64(0), lane);
   __builtin_amdgcn_sched_barrier(0);
-  f4 x[4][2];
+  f4 x[4][kNT64];
 #pragma unroll
   for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-    for (int nt = 0; nt < 2; nt++) x[mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
-  // (every loop has a constant trip count, so all indices fold after unrolling
-  // and the fragment arrays stay in registers)
+    for (int nt = 0; nt < kNT64; nt++) x[mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
+  // X = V^T Zw, window row tile by row tile (every loop has a constant trip
+  // count, so all indices fold after unrolling and the arrays stay in
+  // registers)
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int rt = 0; rt < 8; rt++) {
 #pragma unroll
-    for (int oi = 0; oi < 8; oi++) {
-      const int rt = (oi & 1) ? 4 + (oi >> 1) : 3 - (oi >> 1);  // 3, 4, 2, 5, 1, 6, 0, 7
+    for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
+        for (int nt = 0; nt < kNT64; nt++) {
           if (mt < vlo64(rt) || mt > vhi64(rt)) continue;
           x[mt][nt] = mma(vt[voff64(rt) + mt - vlo64(rt)][i], (rt < 4 ? top[rt & 3][nt] : bot[rt & 3][nt])[i],
                           x[mt][nt]);
         }
+    if (rt == 4) {
+      __builtin_amdgcn_sched_barrier(0);
+      load_tiles<10>(wb, blk, woff64(4), lane);
+      __builtin_amdgcn_sched_barrier(0);
     }
+  }
   __builtin_amdgcn_sched_barrier(0);
-  if (nxt) load_tiles<20>(vt, nxt, 0, lane);
-  __builtin_amdgcn_sched_barrier(0);
+  // Zw += (-V T) X, window row tile by row tile
 #pragma unroll
-  for (int st = 0; st < 4; st++)
+  for (int rt = 0; rt < 8; rt++) {
+    if (rt == 4) {
+      __builtin_amdgcn_sched_barrier(0);
+      if (nxt) load_tiles<20>(vt, nxt, 0, lane);
+      __builtin_amdgcn_sched_barrier(0);
+    }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int st = 0; st < 4; st++)
 #pragma unroll
-      for (int rt = 0; rt < 8; rt++)
+      for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
+        for (int nt = 0; nt < kNT64; nt++) {
           if (st < wlo64(rt)) continue;
-          const f4 &w = wt[woff64(rt) - 20 + st - wlo64(rt)];
+          const int q = woff64(rt) + st - wlo64(rt);
+          const f4 &w = rt < 4 ? wa[(q - woff64(0)) & 15] : wb[(q - woff64(4)) % 10];
           if (rt < 4)
             top[rt & 3][nt] = mma(w[i], x[st][nt][i], top[rt & 3][nt]);
           else
             bot[rt & 3][nt] = mma(w[i], x[st][nt][i], bot[rt & 3][nt]);
         }
+  }
 }
 
 // The b = 64 pipeline: as bcw_kernel (waves over consecutive sweep blocks,
-// rows handed on through LDS), with 32 columns per workgroup and 4 waves, one
-// per SIMD: a wave's window, X and a block's fragments need more than the 256
-// registers two waves per SIMD leave each.  Q is streamed from HBM once per
-// 256 sweeps.  Per step the fragment loads come first and the HBM row traffic
-// after the products, so waiting for a fragment never waits for HBM.
+// rows handed on through LDS), with kC64 = 64 columns per workgroup and 4
+// waves, one per SIMD (a wave's window, X and a block's fragments take
+// about 420 of its 512 registers).  Every workgroup streams all WY blocks,
+// so the column width sets the fragment traffic: 64 columns read the 6.2 GB
+// of blocks (n = 32768) 512 times, half what 32 columns did.  The LDS holds
+// the hand-off slots and wave 0's incoming rows (160 KiB); each wave loads
+// the top half of its first step of a group straight into its registers, two
+// steps ahead.  Q is streamed from HBM once per 256 sweeps.
 __global__ __launch_bounds__(256) void bcw64_kernel(int n, int ncols, float *__restrict__ Z, long ldz,
                                                     const f4 *__restrict__ F, int nsweeps, long z_stride,
                                                     long f_stride) {
   constexpr int NW = 4;
+  constexpr int kQ = HalfIO64::kQ;
   Z += blockIdx.y * z_stride;
   F += blockIdx.y * f_stride;
-  __shared__ f4 xch[NW * 2 * 512];
-  __shared__ f4 stage[2 * 512];
-  __shared__ f4 stage_top[2 * 512];
+  __shared__ f4 xch[NW * 2 * kHalf64];
+  __shared__ f4 stage[2 * kHalf64];
   const int lane = threadIdx.x & 63;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int col0 = blockIdx.x * 32;
+  const int col0 = blockIdx.x * kC64;
   const int nblk = (int)wy64_sweep_blocks(nsweeps);
   const long Jn = (n + kB64 - 1) / kB64;
-  const HalfIO64 io(Z + (long)col0 * ldz, ldz, n, min(32, ncols - col0), lane, k);
-  f4 zt[4][2], zb[4][2];
+  const HalfIO64 io(Z + (long)col0 * ldz, ldz, n, min(kC64, ncols - col0), lane, k);
+  f4 zt[4][kNT64], zb[4][kNT64];
   f4 vt[20];
 
   const int ngroups = (nblk + NW - 1) / NW;
@@ -750,42 +789,37 @@ __global__ __launch_bounds__(256) void bcw64_kernel(int n, int ncols, float *__r
     const int kl = nact - 1;
     const long nst = wy_steps(Jn, bk);
     const f4 *fb = F + wy_prefix(Jn, bk) * (long)(kBlockFrags64 * kFrag);
-    auto body = [&](int tau, float(&lb)[8], float(&lt)[8], const float(&pb)[8], const float(&pt)[8])
-                    __attribute__((always_inline)) {
+    float lb[kQ];
+    for (int tau = -2; tau <= T; tau++) {
       const int e = tau - k;
       const bool act = tau >= 0 && tau < T && k < nact && e >= 0 && e < J;
       const bool app = act && e < nst;
       const f4 *cur = fb + (long)e * (kBlockFrags64 * kFrag);
       if (app && e == 0) load_tiles<20>(vt, cur, 0, lane);
+      // this wave's first top half, two steps ahead (no other wave of the
+      // group touches those rows before then)
+      if (tau == k - 2 && k < nact) io.load_acc(zt, (long)bk * kB64, n, ldz, lane);
+      // wave 0's bottom half for step tau+1: in flight during this step's
+      // products, parked at its end
+      const bool stage_on = tau >= -1 && tau + 1 < J;
+      io.load(lb, stage_on, (long)(gtop + tau + 2) * kB64, n, ldz);
       if (act) {
-        if (e == 0) {
-          get_half64(zt, stage_top + (k & 1) * 512, lane);
-        } else {
+        if (e > 0) {
 #pragma unroll
           for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-            for (int nt = 0; nt < 2; nt++) zt[rt][nt] = zb[rt][nt];
+            for (int nt = 0; nt < kNT64; nt++) zt[rt][nt] = zb[rt][nt];
         }
-        get_half64(zb, k == 0 ? stage + (e & 1) * 512 : xch + ((k - 1) * 2 + (e & 1)) * 512, lane);
-        if (app) apply_block64(zt, zb, vt, cur, (e + 1 < nst && e + 1 < J) ? cur + kBlockFrags64 * kFrag : nullptr, lane);
-        put_half64(xch + (k * 2 + (e & 1)) * 512, zt, lane);
+        get_half64(zb, k == 0 ? stage + (e & 1) * kHalf64 : xch + ((k - 1) * 2 + (e & 1)) * kHalf64, lane);
+        if (app)
+          apply_block64(zt, zb, vt, cur, (e + 1 < nst && e + 1 < J) ? cur + kBlockFrags64 * kFrag : nullptr, lane);
+        put_half64(xch + (k * 2 + (e & 1)) * kHalf64, zt, lane);
       }
       // rows the last wave handed on at step tau-1 -> Q
       const int el = tau - 1 - kl;
-      io.store(xch + (kl * 2 + (el & 1)) * 512, el >= 0 && el < J, (long)(gtop - kl + el) * kB64, n, ldz);
-      // rows needed at step tau+2: wave 0's new bottom half, and the top half
-      // of wave tau+2's first step
-      io.load(lb, tau + 2 < J, (long)(gtop + tau + 3) * kB64, n, ldz);
-      io.load(lt, tau + 2 < nact, (long)(gtop - tau - 2) * kB64, n, ldz);
-      // park the rows loaded at step tau-1 (for step tau+1)
-      if (tau >= -1 && tau + 1 < J) io.park(stage + ((tau + 1) & 1) * 512, pb);
-      if (tau >= -1 && tau + 1 < nact) io.park(stage_top + ((tau + 1) & 1) * 512, pt);
+      io.store(xch + (kl * 2 + (el & 1)) * kHalf64, el >= 0 && el < J, (long)(gtop - kl + el) * kB64, n, ldz);
+      if (stage_on) io.park(stage + ((tau + 1) & 1) * kHalf64, lb);
       lds_barrier();
-    };
-    float la[8], ta[8], lb2[8], tb2[8];
-    for (int tau = -2; tau <= T; tau += 2) {
-      body(tau, la, ta, lb2, tb2);
-      if (tau + 1 <= T) body(tau + 1, lb2, tb2, la, ta);
     }
     wait_mem();
     __syncthreads();
@@ -844,8 +878,8 @@ size_t bc_back_wy64_bytes(long n) {
                   sizeof(f4));
 }
 
-// the 32 columns of a workgroup must be addressable with 32-bit buffer offsets
-bool bc_back_wy64_fits(long n, long ldq) { return (31 * ldq + n) * 4 < (1L << 31); }
+// the columns of a workgroup must be addressable with 32-bit buffer offsets
+bool bc_back_wy64_fits(long n, long ldq) { return ((kC64 - 1) * ldq + n) * 4 < (1L << 31); }
 
 // Q <- H_1 .. H_K Q for b = 64 as WY blocks on FP32 MFMA (every precision
 // form), a batch of `count` same-size problems per launch.
@@ -863,7 +897,7 @@ int bc_back_wy64_batched(evd_ctx *ctx, int count, long n, long ncols, float *Q, 
       (int)n, U, ldu, nsweeps, F, u_stride, f_stride);
   EVD_LAUNCHED(ctx);
   if (ctx->timing) hipEventRecord(ctx->kev[0], ctx->stream);
-  bcw64_kernel<<<dim3((unsigned)((ncols + 31) / 32), (unsigned)count), 256, 0, ctx->stream>>>(
+  bcw64_kernel<<<dim3((unsigned)((ncols + kC64 - 1) / kC64), (unsigned)count), 256, 0, ctx->stream>>>(
       (int)n, (int)ncols, Q, ldq, F, nsweeps, q_stride, f_stride);
   EVD_LAUNCHED(ctx);
   if (ctx->timing) hipEventRecord(ctx->kev[1], ctx->stream);
