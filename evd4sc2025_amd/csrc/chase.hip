@@ -151,23 +151,28 @@ __global__ __launch_bounds__(256, BW == 32 ? 4 : 1) void chase_kernel(int n, int
         Sr[r] = (ri < len && kk < len) ? ld_band(rb, (unsigned)dd + (unsigned)(p + lo) * ld) : 0.0;
         Nr[r] = (ri < len2 && kk < len) ? ld_band(rb, (unsigned)(len + ri - kk) + (unsigned)(p + kk) * ld) : 0.0;
       }
-      // ---- B1 <- H B1 and write-back (wave 3, while the loads are in flight)
+      // ---- B1 <- H B1 and write-back (wave 3, while the loads are in flight):
+      // t_c = u^T B1[:, c] by lane c (rows split over HALVES lane sets), then
+      // the write-back with lanes along rows, so each store instruction
+      // covers whole runs of one or two band columns instead of one element
+      // of 32 or 64 columns
       if (wave == 3) {
         const int c = lane % BW, h = lane / BW;
         double t = 0.0;
 #pragma unroll
         for (int i = 0; i < BW; i += HALVES) t += u[i + h] * B1[(i + h) * LD + c];
         if (HALVES == 2) t += __shfl_xor(t, 32, 64);
-        if (c < colB1) {
-          const double beta = s_beta;
-          const int col = cB1 + c;
+        const double beta = s_beta;
+        const int ii = lane % BW;
+        const double uii = u[ii];
 #pragma unroll
-          for (int i = 0; i < BW; i += HALVES) {
-            const int ii = i + h;
-            if (ii < len && !(c == 0 && ii == 0 && !first)) {
-              const double v = (c == 0) ? (ii == 0 ? beta : 0.0) : B1[ii * LD + c] - u[ii] * t;
-              st_band(rb, (unsigned)(p + ii - col) + (unsigned)col * ld, v);
-            }
+        for (int c0 = 0; c0 < BW; c0 += HALVES) {
+          const int cc = c0 + h;
+          const double tc = __shfl(t, cc, 64);
+          if (cc < colB1 && ii < len && !(cc == 0 && ii == 0 && !first)) {
+            const int col = cB1 + cc;
+            const double v = (cc == 0) ? (ii == 0 ? beta : 0.0) : B1[ii * LD + cc] - uii * tc;
+            st_band(rb, (unsigned)(p + ii - col) + (unsigned)col * ld, v);
           }
         }
       }
