@@ -437,12 +437,59 @@ static bool use_cholqr(const evd_ctx *ctx, long m, long nc) {
   return !ctx->tune.panel_exchange && nc <= 32 && m >= 2 * nc;
 }
 
+// Panels of 33..64 FP32 columns (b = 64) as two halves on the Cholesky QR
+// path: Q = Q1 Q2 with Q1 from the first 32 columns, the rest updated by
+// Q1^T and factored from row 32 on.  In WY form
+//   V = [V1, [0; V2]],  W = V T = [W1, W2 - W1 (V1^T W2)],
+//   R = [R11 R12; 0 R22],  R12 = the top 32 rows of Q1^T A2,
+// which is what factoring column by column gives (same reflector signs),
+// within rounding.
+static bool use_cholqr_halves(const evd_ctx *ctx, long m, long nc) {
+  return !ctx->tune.panel_exchange && nc > 32 && nc <= 64 && m >= 2 * nc;
+}
+
+template <typename T>
+int panel_qr_t(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *Y, long ldy, T *W, long ldw, T *R, long ldr,
+               T *Aout, long ldo, double *tau_out);
+
+// Single panel: A (read) may alias Y or Aout, as in the callers.
+static int panel_halves(evd_ctx *ctx, long m, long nc, const float *A, long lda, float *Y, long ldy, float *W,
+                        long ldw, float *R, long ldr, float *Aout, long ldo, double *tau_out) {
+  const long n2 = nc - 32;
+  ArenaFrame frame(ctx);
+  float *P2 = arena_alloc_n<float>(ctx, (size_t)m * n2);
+  float *X = arena_alloc_n<float>(ctx, (size_t)32 * n2);
+  if (!P2 || !X) return EVD_E_NOMEM;
+  EVD_TRY(copy_f32(ctx, A + 32 * lda, lda, P2, m, m, n2));
+  EVD_TRY(panel_qr_t<float>(ctx, m, 32, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out));
+  // A2 <- Q1^T A2 = A2 - V1 (W1^T A2)
+  EVD_TRY(sgemm(ctx, 'T', 'N', 32, n2, m, 1.f, W, ldw, P2, m, 0.f, X, 32));
+  EVD_TRY(sgemm(ctx, 'N', 'N', m, n2, 32, -1.f, Y, ldy, X, 32, 1.f, P2, m));
+  // the second half from row 32 ([R22; 0] back into P2)
+  EVD_TRY(panel_qr_t<float>(ctx, m - 32, n2, P2 + 32, m, Y + 32 + 32 * ldy, ldy, W + 32 + 32 * ldw, ldw,
+                            R ? R + 32 + 32 * ldr : nullptr, ldr, P2 + 32, m, tau_out ? tau_out + 32 : nullptr));
+  if (Aout) EVD_TRY(copy_f32(ctx, P2, m, Aout + 32 * ldo, ldo, m, n2));
+  if (R) {
+    EVD_TRY(copy_f32(ctx, P2, m, R + 32 * ldr, ldr, 32, n2));
+    EVD_TRY(fill_f32(ctx, R + 32, n2, 32, ldr, 0.f));
+  }
+  EVD_TRY(fill_f32(ctx, Y + 32 * ldy, 32, n2, ldy, 0.f));
+  EVD_TRY(fill_f32(ctx, W + 32 * ldw, 32, n2, ldw, 0.f));
+  // W2 <- W2 - W1 (V1^T W2) (rows 0..31 of W2 are zero)
+  EVD_TRY(sgemm(ctx, 'T', 'N', 32, n2, m - 32, 1.f, Y + 32, ldy, W + 32 + 32 * ldw, ldw, 0.f, X, 32));
+  EVD_TRY(sgemm(ctx, 'N', 'N', m, n2, 32, -1.f, W, ldw, X, 32, 1.f, W + 32 * ldw, ldw));
+  return 0;
+}
+
 // Panel factorisation used by the SBR drivers (FP32 and FP64 storage; the
 // arithmetic is FP64 in both) and by evd_panel_qr_f32 / _f64.
 template <typename T>
 int panel_qr_t(evd_ctx *ctx, long m, long nc, const T *A, long lda, T *Y, long ldy, T *W, long ldw, T *R, long ldr,
                T *Aout, long ldo, double *tau_out) {
   if (m <= 0 || nc <= 0) return 0;
+  if constexpr (sizeof(T) == 4) {
+    if (use_cholqr_halves(ctx, m, nc)) return panel_halves(ctx, m, nc, A, lda, Y, ldy, W, ldw, R, ldr, Aout, ldo, tau_out);
+  }
   if (use_cholqr(ctx, m, nc)) {
     // Cholesky QR + reconstruction; the column-exchange kernel runs after it
     // and factors the panel only if it was flagged
@@ -479,6 +526,29 @@ int panel_qr_f64(evd_ctx *ctx, long m, long nc, const double *A, long lda, doubl
 int panel_qr_batched(evd_ctx *ctx, int count, long m, long nc, float *A, long lda, long sA, float *Y, long ldy,
                      long sY, float *W, long ldw, long sW) {
   if (m <= 0 || nc <= 0 || count <= 0) return 0;
+  if (use_cholqr_halves(ctx, m, nc)) {
+    // panel_halves for every problem, each step one launch for all of them
+    const long n2 = nc - 32;
+    ArenaFrame frame(ctx);
+    float *P2 = arena_alloc_n<float>(ctx, (size_t)count * m * n2);
+    float *X = arena_alloc_n<float>(ctx, (size_t)count * 32 * n2);
+    if (!P2 || !X) return EVD_E_NOMEM;
+    const long sP = m * n2, sX = 32 * n2;
+    for (int q = 0; q < count; q++) EVD_TRY(copy_f32(ctx, A + q * sA + 32 * lda, lda, P2 + q * sP, m, m, n2));
+    EVD_TRY(panel_qr_batched(ctx, count, m, 32, A, lda, sA, Y, ldy, sY, W, ldw, sW));
+    EVD_TRY(sgemm(ctx, 'T', 'N', 32, n2, m, 1.f, W, ldw, P2, m, 0.f, X, 32, count, sW, sP, sX));
+    EVD_TRY(sgemm(ctx, 'N', 'N', m, n2, 32, -1.f, Y, ldy, X, 32, 1.f, P2, m, count, sY, sX, sP));
+    EVD_TRY(panel_qr_batched(ctx, count, m - 32, n2, P2 + 32, m, sP, Y + 32 + 32 * ldy, ldy, sY, W + 32 + 32 * ldw,
+                             ldw, sW));
+    for (int q = 0; q < count; q++) {
+      EVD_TRY(copy_f32(ctx, P2 + q * sP, m, A + q * sA + 32 * lda, lda, m, n2));
+      EVD_TRY(fill_f32(ctx, Y + q * sY + 32 * ldy, 32, n2, ldy, 0.f));
+      EVD_TRY(fill_f32(ctx, W + q * sW + 32 * ldw, 32, n2, ldw, 0.f));
+    }
+    EVD_TRY(sgemm(ctx, 'T', 'N', 32, n2, m - 32, 1.f, Y + 32, ldy, W + 32 + 32 * ldw, ldw, 0.f, X, 32, count, sY, sW,
+                  sX));
+    return sgemm(ctx, 'N', 'N', m, n2, 32, -1.f, W, ldw, X, 32, 1.f, W + 32 * ldw, ldw, count, sW, sX, sW);
+  }
   if (use_cholqr(ctx, m, nc)) {
     // every problem's panel through the Cholesky QR path in one launch each,
     // then the fallback for the flagged ones (one launch when the groups of
