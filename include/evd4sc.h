@@ -88,8 +88,9 @@ int evd_ctx_set_tuning(evd_ctx *ctx, int bc_sweeps, int bc_waves, int chase_grou
  * plain FP32 products run on this library's MFMA kernels; by default they go
  * to rocBLAS on the context stream, deterministic mode) and "panel_exchange"
  * (1: every panel on the column-exchange kernel; by default panels of up to
- * 32 columns take the blocked Cholesky QR path, with the exchange kernel as
- * its on-device fallback).  Returns -2 for an unknown name, -3 for a value
+ * 32 columns take the blocked Cholesky QR path, and FP32 panels of 33..64
+ * columns take it as two 32-column halves, with the exchange kernel as its
+ * on-device fallback).  Returns -2 for an unknown name, -3 for a value
  * out of range.
  * No counterpart in the reference (its tile shapes are compile-time). */
 int evd_ctx_set_knob(evd_ctx *ctx, const char *name, int value);
