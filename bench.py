@@ -93,8 +93,8 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     LAPACKE_dstedc on the (D, E) the GPU chase produced for the headline
     matrix (the reference's own host stage, src/EVD/myEVD.cu:529) and, with
     full=True, MKL LAPACKE_dsyevd (jobz=V) on the headline matrix itself;
-    without it, dsyevd on the n = 8192 matrix of the same workload family
-    (uniform, seed 0) as a bounded sample.  value = (10/3 n^3) / seconds of
+    without it, dsyevd on the n = 5120 matrix of the same workload family
+    (uniform, seed 0) as a bounded sample (8-30 s on the hosts measured).  value = (10/3 n^3) / seconds of
     the dsyevd run, the metric's normalisation."""
     import ctypes
 
@@ -113,7 +113,7 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     else:
         from oracle import evd_oracle as O
 
-        nn = 8192
+        nn = 5120
         A = np.asfortranarray(O.gen_symmetric(nn, 0, 0).astype(np.float64))
     w = np.empty(nn)
     t0 = time.perf_counter()
