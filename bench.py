@@ -92,11 +92,11 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     """Host LAPACK on the headline workload, on this machine's cores: MKL
     LAPACKE_dstedc on the (D, E) the GPU chase produced for the headline
     matrix (the reference's own host stage, src/EVD/myEVD.cu:529) and, with
-    full=True, MKL LAPACKE_dsyevd (jobz=V) on the headline matrix itself;
-    without it, dsyevd on the n = 5120 matrix of the same workload family
-    (uniform, seed 0) as a bounded sample (8-30 s on the hosts measured).
-    Where MKL fails on the host, the SciPy build's LAPACK takes its place and
-    the failure is recorded.  value = (10/3 n^3) / seconds of
+    full=True, LAPACK dsyevd (jobz=V) on the headline matrix itself; without
+    it, dsyevd on the n = 8192 matrix of the same workload family (uniform,
+    seed 0) as a bounded sample.  dsyevd comes from the SciPy build (OpenBLAS),
+    with MKL as the fallback; dstedc from MKL, with the SciPy build as the
+    fallback; each failure is recorded.  value = (10/3 n^3) / seconds of
     the dsyevd run, the metric's normalisation."""
     import ctypes
 
@@ -115,32 +115,31 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     else:
         from oracle import evd_oracle as O
 
-        nn = 5120
+        nn = 8192
         A = np.asfortranarray(O.gen_symmetric(nn, 0, 0).astype(np.float64))
-    w = np.empty(nn)
-    A_keep = np.array(A, order="F")  # for the fallback below (dsyevd overwrites its input)
-    t0 = time.perf_counter()
-    rc = mkl.LAPACKE_dsyevd(102, ctypes.c_char(b"V"), ctypes.c_char(b"L"), nn, dp(A), nn, dp(w))
-    t_syevd = time.perf_counter() - t0
-    del A
-    syevd_lib, syevd_err = "MKL LAPACKE_dsyevd", {}
-    if rc != 0:
-        # MKL's divide and conquer has failed on some hosts' CPUs (a leaf
-        # solve, info > 0); LAPACK dsyevd from the SciPy build on the same
-        # matrix then, and the MKL failure is recorded beside it
-        syevd_err = {"host_dsyevd_mkl_error": f"LAPACKE_dsyevd returned {rc} at n={nn} after {t_syevd:.2f} s"}
-        try:
-            from scipy.linalg import lapack
+    # the dsyevd sample: LAPACK from the SciPy build (OpenBLAS) first; MKL's
+    # dsyevd was 40x slower on a GPU box's host CPU and then failed there
+    # (a divide-and-conquer leaf, info > 0), so it is only the fallback
+    syevd_lib, syevd_err = "SciPy LAPACK dsyevd (OpenBLAS)", {}
+    try:
+        from scipy.linalg import lapack
 
-            t0 = time.perf_counter()
-            _, _, info = lapack.dsyevd(A_keep, compute_v=1, lower=1, overwrite_a=1)
-            t_syevd = time.perf_counter() - t0
-        except Exception as ex:  # noqa: BLE001 - reported in the line
-            return {"value": None, "error": syevd_err["host_dsyevd_mkl_error"] + f"; SciPy dsyevd: {ex!r}"}
+        t0 = time.perf_counter()
+        _, _, info = lapack.dsyevd(A, compute_v=1, lower=1, overwrite_a=0)
+        t_syevd = time.perf_counter() - t0
         if info != 0:
-            return {"value": None, "error": syevd_err["host_dsyevd_mkl_error"] + f"; SciPy dsyevd returned {info}"}
-        syevd_lib = "SciPy LAPACK dsyevd (OpenBLAS)"
-    del A_keep
+            syevd_err = {"host_dsyevd_scipy_error": f"SciPy dsyevd returned {info} at n={nn}"}
+    except Exception as ex:  # noqa: BLE001 - reported in the line
+        syevd_err = {"host_dsyevd_scipy_error": repr(ex)}
+    if syevd_err:
+        w = np.empty(nn)
+        t0 = time.perf_counter()
+        rc = mkl.LAPACKE_dsyevd(102, ctypes.c_char(b"V"), ctypes.c_char(b"L"), nn, dp(A), nn, dp(w))
+        t_syevd = time.perf_counter() - t0
+        if rc != 0:
+            return {"value": None, "error": f"{syevd_err}; LAPACKE_dsyevd returned {rc} at n={nn}"}
+        syevd_lib = "MKL LAPACKE_dsyevd"
+    del A
     # the reference's own host stage on the headline (D, E): MKL dstedc; if
     # MKL reports a failure on this host, LAPACK dsbevd on the same matrix
     # stored as a band of width 1 (its tridiagonal stage is dstedc) from the
