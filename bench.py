@@ -83,8 +83,8 @@ def _mkl():
 
 
 def host_threads() -> int:
-    """Threads the host LAPACK runs on: OMP_NUM_THREADS when set (the GPU
-    box's CPU share), else the machine's core count."""
+    """Threads the host LAPACK runs on: OMP_NUM_THREADS when set (the CPU
+    share the node gives this process), else the machine's core count."""
     return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
 
 
@@ -118,7 +118,7 @@ def cpu_baseline(A_host, D, Ed, full: bool):
         nn = 8192
         A = np.asfortranarray(O.gen_symmetric(nn, 0, 0).astype(np.float64))
     # the dsyevd sample: LAPACK from the SciPy build (OpenBLAS) first; MKL's
-    # dsyevd was 40x slower on a GPU box's host CPU and then failed there
+    # dsyevd was 40x slower on the host CPU of an MI355X node and then failed there
     # (a divide-and-conquer leaf, info > 0), so it is only the fallback
     syevd_lib, syevd_err = "SciPy LAPACK dsyevd (OpenBLAS)", {}
     try:
