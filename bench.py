@@ -96,7 +96,8 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     it, dsyevd on the n = 8192 matrix of the same workload family (uniform,
     seed 0) as a bounded sample.  dsyevd comes from the SciPy build (OpenBLAS),
     with MKL as the fallback; dstedc from MKL, with the SciPy build as the
-    fallback; each failure is recorded.  value = (10/3 n^3) / seconds of
+    fallback (MKL is optional: without it only the SciPy legs run); each
+    failure is recorded.  value = (10/3 n^3) / seconds of
     the dsyevd run, the metric's normalisation."""
     import ctypes
 
@@ -104,9 +105,8 @@ def cpu_baseline(A_host, D, Ed, full: bool):
 
     mkl = _mkl()
     threads = host_threads()
-    if mkl is None:
-        return {"value": None, "error": "no MKL runtime (libmkl_rt) on this host"}
-    mkl.MKL_Set_Num_Threads(ctypes.c_int(threads))
+    if mkl is not None:
+        mkl.MKL_Set_Num_Threads(ctypes.c_int(threads))
     dp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
     n = D.shape[0]
     if full:
@@ -132,6 +132,8 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     except Exception as ex:  # noqa: BLE001 - reported in the line
         syevd_err = {"host_dsyevd_scipy_error": repr(ex)}
     if syevd_err:
+        if mkl is None:
+            return {"value": None, "error": f"{syevd_err}; no MKL runtime (libmkl_rt) on this host"}
         w = np.empty(nn)
         t0 = time.perf_counter()
         rc = mkl.LAPACKE_dsyevd(102, ctypes.c_char(b"V"), ctypes.c_char(b"L"), nn, dp(A), nn, dp(w))
@@ -149,13 +151,14 @@ def cpu_baseline(A_host, D, Ed, full: bool):
     e = np.array(Ed, dtype=np.float64)
     Z = np.empty((n, n), dtype=np.float64, order="F")
     t0 = time.perf_counter()
-    rc = mkl.LAPACKE_dstedc(102, ctypes.c_char(b"I"), n, dp(d), dp(e), dp(Z), n)
+    rc = mkl.LAPACKE_dstedc(102, ctypes.c_char(b"I"), n, dp(d), dp(e), dp(Z), n) if mkl is not None else None
     t_stedc = time.perf_counter() - t0
     del Z
     if rc == 0:
         stedc.update({"host_dstedc_s": t_stedc, "host_dstedc_lib": "MKL LAPACKE_dstedc"})
     else:
-        stedc["host_dstedc_mkl_error"] = f"LAPACKE_dstedc returned {rc} at n={n} after {t_stedc:.2f} s"
+        stedc["host_dstedc_mkl_error"] = (f"LAPACKE_dstedc returned {rc} at n={n} after {t_stedc:.2f} s"
+                                          if mkl is not None else "no MKL runtime (libmkl_rt) on this host")
         try:
             from scipy.linalg import lapack
 
